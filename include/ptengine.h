@@ -1,0 +1,192 @@
+/*
+ * ptengine.h -- C ABI of the MI355X process-tensor engine (libptengine.so).
+ *
+ * The engine replaces the tensor-network backends of OQuPy's process-tensor
+ * path. Every entry point below names the reference interface it replaces
+ * (paths relative to the OQuPy 0.5.0 source tree). Conventions:
+ *
+ *  - complex numbers are interleaved (re, im) doubles, as numpy complex128;
+ *  - tensors are row-major, with their shape passed explicitly;
+ *  - the caller owns every host buffer; the engine copies inputs in and writes
+ *    outputs into caller buffers, and owns its device memory until *_destroy;
+ *  - every function returns 0 on success and a negative PTE_ERR_* code on
+ *    failure; pte_last_error() returns the message of the last failure on the
+ *    calling thread;
+ *  - calls on one handle must be serialised by the caller; the engine makes no
+ *    callbacks.
+ */
+#ifndef PTENGINE_H
+#define PTENGINE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PTE_OK 0
+#define PTE_ERR_ARG (-1)       /* invalid argument (shape, index, value)     */
+#define PTE_ERR_STATE (-2)     /* call out of order (e.g. step before init)  */
+#define PTE_ERR_DEVICE (-3)    /* HIP runtime error                          */
+#define PTE_ERR_CAPACITY (-4)  /* a bond would exceed the configured capacity */
+#define PTE_ERR_NUMERIC (-5)   /* non-finite input or SVD not converged      */
+
+#define PTE_PRECISION_C128 0
+
+typedef struct pte_pt_engine pte_pt_engine;
+typedef struct pte_tempo_engine pte_tempo_engine;
+
+/* Configuration of a PT-TEMPO computation.
+ * Replaces PtTempoBackend.__init__ (oqupy/backends/pt_tempo_backend.py:56-93). */
+typedef struct {
+  int dimension;    /* Hilbert space dimension d                              */
+  int num_steps;    /* N, number of time steps of the process tensor (>= 2)   */
+  int dkmax;        /* memory cut K (the backend's dkmax, never None)         */
+  double epsrel;    /* relative SVD truncation error                          */
+  int precision;    /* PTE_PRECISION_C128                                     */
+  int chi_cap;      /* capacity of every bond; exceeding it is an error       */
+  int n_north;      /* length of sum_north (d^2, or the degeneracy classes)   */
+} pte_pt_config;
+
+/* Create an engine; *out receives the handle. */
+int pte_pt_create(const pte_pt_config* cfg, pte_pt_engine** out);
+
+/* Influence tensor of distance dk (0 <= dk < min(N, K+1)), as returned by
+ * influence_matrix (oqupy/tempo.py:969-1020): rows x cols complex values;
+ * cols == 0 marks a vector of length rows (dk = 0 with degeneracy maps).
+ * Replaces the influence callable given to PtTempoBackend. */
+int pte_pt_set_influence(pte_pt_engine* e, int dk, const double* data, int rows,
+                         int cols);
+
+/* Degeneracy maps of Bath (oqupy/bath.py:87-89), each of length d^2; only for
+ * unique=True (pt_tempo_backend.py:114-140). */
+int pte_pt_set_degeneracy_maps(pte_pt_engine* e, const int* north_map,
+                               const int* west_map, int length);
+
+/* Build the influence MPO and the initial MPS and run the two truncating
+ * sweeps. Replaces PtTempoBackend.initialize (pt_tempo_backend.py:105-190). */
+int pte_pt_initialize(pte_pt_engine* e);
+
+/* One PT-TEMPO step. `last_influence` is influence(-step) (rows x cols) when a
+ * correlation time is added, else NULL. *more is set to step < num_steps.
+ * Replaces PtTempoBackend.compute_step (pt_tempo_backend.py:192-282). */
+int pte_pt_step(pte_pt_engine* e, const double* last_influence, int rows,
+                int cols, int* more);
+
+/* Current step (0 before initialisation). */
+int pte_pt_get_step(pte_pt_engine* e, int* step);
+
+/* Number of sites of the MPS and its bond dimensions (out has n_sites-1
+ * entries). */
+int pte_pt_num_sites(pte_pt_engine* e, int* n_sites);
+int pte_pt_bond_dims(pte_pt_engine* e, int* out, int n);
+
+/* PT-MPO tensor of time step `step` in the layout [bond_past, bond_future,
+ * d^2], multiplied by d, into out (capacity in complex values `cap`); shape3
+ * receives its shape. Replaces PtTempoBackend.get_mpo_tensor
+ * (pt_tempo_backend.py:284-307). */
+int pte_pt_get_mpo_tensor(pte_pt_engine* e, int step, double* out, long cap,
+                          int* shape3);
+
+/* Caps of the process tensor, computed on the device: caps[k] has length
+ * bond_past of step k (caps[N] = [1]). Replaces SimpleProcessTensor.compute_caps
+ * (oqupy/process_tensor.py:380-406). pte_pt_get_cap copies caps[k]. */
+int pte_pt_compute_caps(pte_pt_engine* e);
+int pte_pt_get_cap(pte_pt_engine* e, int k, double* out, int cap, int* length);
+
+/* Statistics of the last step: number of SVDs, total Jacobi sweeps, largest
+ * bond, accumulated device time of the step in milliseconds. */
+int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps,
+                           int* max_bond, double* ms);
+
+void pte_pt_destroy(pte_pt_engine* e);
+
+/* ---- TEMPO (oqupy/backends/tempo_backend.py:314-626) ---------------------- */
+
+typedef struct {
+  int dimension;    /* Hilbert space dimension d                              */
+  int dkmax;        /* memory cut K, or -1 for no cut (dkmax None)            */
+  double epsrel;
+  int precision;
+  int chi_cap;
+  int n_north;      /* length of sum_north                                    */
+  int n_west;       /* length of sum_west                                     */
+} pte_tempo_config;
+
+/* Replaces BaseTempoBackend.__init__ (tempo_backend.py:343-372). */
+int pte_tempo_create(const pte_tempo_config* cfg, pte_tempo_engine** out);
+
+/* Influence of distance dk >= 0 (see pte_pt_set_influence). For dkmax = -1
+ * the caller sets dk = len(MPO) before each step, as the reference grows the
+ * MPO (tempo_backend.py:487-497). */
+int pte_tempo_set_influence(pte_tempo_engine* e, int dk, const double* data,
+                            int rows, int cols);
+
+/* Degeneracy maps for unique=True (tempo_backend.py:400-417). */
+int pte_tempo_set_degeneracy_maps(pte_tempo_engine* e, const int* north_map,
+                                  const int* west_map, int length);
+
+/* Initial state vector (d^2) and the unitary of the coupling's eigenbasis
+ * (d x d). Replaces BaseTempoBackend.initialize_mps_mpo
+ * (tempo_backend.py:379-437). */
+int pte_tempo_initialize(pte_tempo_engine* e, const double* initial_state,
+                         const double* unitary);
+
+/* One TEMPO step with the half-step propagators prop_1, prop_2 (d^2 x d^2) and,
+ * after the memory is full, influence(dkmax - step) (rows x cols) or NULL.
+ * Writes the new state vector (d^2) into state_out. Replaces
+ * BaseTempoBackend.compute_system_step (tempo_backend.py:439-575). */
+int pte_tempo_step(pte_tempo_engine* e, const double* prop_1,
+                   const double* prop_2, const double* extra_influence,
+                   int rows, int cols, double* state_out);
+
+int pte_tempo_get_step(pte_tempo_engine* e, int* step);
+int pte_tempo_bond_dims(pte_tempo_engine* e, int* out, int n, int* n_sites);
+void pte_tempo_destroy(pte_tempo_engine* e);
+
+/* ---- compute_dynamics with one process tensor ------------------------------
+ * Replaces the step loop of compute_dynamics (oqupy/system_dynamics.py:41-182,
+ * _apply_system_superoperator 631-640, _apply_caps 643-651, _apply_pt_mpos
+ * 654-700) for a single environment without controls.
+ *   d2          Liouville dimension
+ *   num_steps   N
+ *   bonds       N+1 bond dimensions (bonds[0] = bonds[N] = 1)
+ *   tensors     concatenated rank-3 PT tensors [bonds[k], bonds[k+1], d2]
+ *   caps        concatenated caps (lengths bonds[k], k = 0..N)
+ *   props       N pairs (P1, P2) of d2 x d2 propagators
+ *   transform_in/out  d2 x d2 basis transforms or NULL (process_tensor.py:349-355)
+ *   rho0        initial state vector (d2)
+ *   states_out  (N+1) x d2 state vectors
+ */
+int pte_dynamics_run(int d2, int num_steps, const int* bonds,
+                     const double* tensors, const double* caps,
+                     const double* props, const double* transform_in,
+                     const double* transform_out, const double* rho0,
+                     double* states_out);
+
+/* Same, reading the PT-MPO tensors and caps held by a finished PT-TEMPO
+ * engine (after pte_pt_compute_caps), so the process tensor never leaves the
+ * device. */
+int pte_dynamics_run_engine(pte_pt_engine* e, const double* props,
+                            const double* transform_in,
+                            const double* transform_out, const double* rho0,
+                            double* states_out);
+
+/* Truncated SVD of one matrix by the engine's kernels: M (m x n, row-major)
+ * = U diag(S) Vh with the kept rank k = #{j : sqrt(sum_{i>=j} s_i^2) >
+ * epsrel * s_0}; U is m x min(m,n) capacity (row-major m x k on return),
+ * S has min(m,n) capacity, SVh receives diag(S) Vh (row-major k x n).
+ * Replaces tensornetwork's split_node_full_svd(max_truncation_err=epsrel,
+ * relative=True) as called at oqupy/backends/node_array.py:262, 285, 541. */
+int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap,
+                  double* U, double* S, double* SVh, int* k_out,
+                  int* sweeps_out);
+
+const char* pte_last_error(void);
+
+/* Library version string. */
+const char* pte_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTENGINE_H */

@@ -1,0 +1,202 @@
+"""ctypes binding of ``libptengine.so``, the MI355X process-tensor engine.
+
+The library is built in-tree (``python -m oqupy_amd.build``) next to this
+package; loading fails loudly when it is missing or when no GPU is visible --
+there is no CPU fallback. Errors returned by the C ABI are mapped to the
+exception types the reference raises for the same misuse: ``ValueError`` for
+bad arguments, ``AssertionError`` for calls out of order, ``RuntimeError``
+for device, capacity and numerical failures.
+"""
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_LIB_NAME = "libptengine.so"
+_LIB = None
+
+PTE_OK = 0
+PTE_ERR_ARG = -1
+PTE_ERR_STATE = -2
+PTE_ERR_DEVICE = -3
+PTE_ERR_CAPACITY = -4
+PTE_ERR_NUMERIC = -5
+
+
+class EngineError(RuntimeError):
+    """A device, capacity or numerical failure reported by the engine."""
+
+    def __init__(self, code: int, message: str) -> None:
+        super().__init__(f"ptengine error {code}: {message}")
+        self.code = code
+
+
+class _PtConfig(ctypes.Structure):
+    _fields_ = [("dimension", ctypes.c_int), ("num_steps", ctypes.c_int),
+                ("dkmax", ctypes.c_int), ("epsrel", ctypes.c_double),
+                ("precision", ctypes.c_int), ("chi_cap", ctypes.c_int),
+                ("n_north", ctypes.c_int)]
+
+
+class _TempoConfig(ctypes.Structure):
+    _fields_ = [("dimension", ctypes.c_int), ("dkmax", ctypes.c_int),
+                ("epsrel", ctypes.c_double), ("precision", ctypes.c_int),
+                ("chi_cap", ctypes.c_int), ("n_north", ctypes.c_int),
+                ("n_west", ctypes.c_int)]
+
+
+def library_path() -> str:
+    """Absolute path of the in-tree engine library."""
+    return os.path.join(os.path.dirname(os.path.dirname(
+        os.path.abspath(__file__))), _LIB_NAME)
+
+
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+_SIGNATURES = {
+    "pte_last_error": (ctypes.c_char_p, []),
+    "pte_version": (ctypes.c_char_p, []),
+    "pte_pt_create": (ctypes.c_int, [ctypes.POINTER(_PtConfig),
+                                     ctypes.POINTER(_P)]),
+    "pte_pt_set_influence": (ctypes.c_int, [_P, ctypes.c_int, _DP,
+                                            ctypes.c_int, ctypes.c_int]),
+    "pte_pt_set_degeneracy_maps": (ctypes.c_int, [_P, _IP, _IP,
+                                                  ctypes.c_int]),
+    "pte_pt_initialize": (ctypes.c_int, [_P]),
+    "pte_pt_step": (ctypes.c_int, [_P, _DP, ctypes.c_int, ctypes.c_int, _IP]),
+    "pte_pt_get_step": (ctypes.c_int, [_P, _IP]),
+    "pte_pt_num_sites": (ctypes.c_int, [_P, _IP]),
+    "pte_pt_bond_dims": (ctypes.c_int, [_P, _IP, ctypes.c_int]),
+    "pte_pt_get_mpo_tensor": (ctypes.c_int, [_P, ctypes.c_int, _DP,
+                                             ctypes.c_long, _IP]),
+    "pte_pt_compute_caps": (ctypes.c_int, [_P]),
+    "pte_pt_get_cap": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int,
+                                      _IP]),
+    "pte_pt_last_step_stats": (ctypes.c_int, [_P, _IP, _IP, _IP, _DP]),
+    "pte_pt_destroy": (None, [_P]),
+    "pte_tempo_create": (ctypes.c_int, [ctypes.POINTER(_TempoConfig),
+                                        ctypes.POINTER(_P)]),
+    "pte_tempo_set_influence": (ctypes.c_int, [_P, ctypes.c_int, _DP,
+                                               ctypes.c_int, ctypes.c_int]),
+    "pte_tempo_set_degeneracy_maps": (ctypes.c_int, [_P, _IP, _IP,
+                                                     ctypes.c_int]),
+    "pte_tempo_initialize": (ctypes.c_int, [_P, _DP, _DP]),
+    "pte_tempo_step": (ctypes.c_int, [_P, _DP, _DP, _DP, ctypes.c_int,
+                                      ctypes.c_int, _DP]),
+    "pte_tempo_get_step": (ctypes.c_int, [_P, _IP]),
+    "pte_tempo_bond_dims": (ctypes.c_int, [_P, _IP, ctypes.c_int, _IP]),
+    "pte_tempo_destroy": (None, [_P]),
+    "pte_dynamics_run": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _IP, _DP,
+                                        _DP, _DP, _DP, _DP, _DP, _DP]),
+    "pte_dynamics_run_engine": (ctypes.c_int, [_P, _DP, _DP, _DP, _DP, _DP]),
+    "pte_split_svd": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _DP,
+                                     ctypes.c_double, ctypes.c_int, _DP, _DP,
+                                     _DP, _IP, _IP]),
+}
+
+
+def exported_symbols():
+    """Names of every C entry point this binding expects."""
+    return sorted(_SIGNATURES)
+
+
+def load_library(path: Optional[str] = None):
+    """Load (once) and return the engine library with typed signatures."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    lib_path = path or library_path()
+    if not os.path.exists(lib_path):
+        raise ImportError(
+            f"The MI355X engine library {lib_path} is not built; run "
+            "`python -m oqupy_amd.build` (hipcc, gfx950).")
+    lib = ctypes.CDLL(lib_path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(code: int) -> None:
+    """Raise the exception matching a non-zero engine return code."""
+    if code == PTE_OK:
+        return
+    msg = load_library().pte_last_error().decode(errors="replace")
+    if code == PTE_ERR_ARG:
+        raise ValueError(msg)
+    if code == PTE_ERR_STATE:
+        raise AssertionError(msg)
+    raise EngineError(code, msg)
+
+
+def require_gpu() -> None:
+    """Fail loudly when no HIP device is visible (no CPU fallback)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("The MI355X engine needs a visible HIP device; "
+                           "none is available.")
+
+
+def as_c128(array) -> np.ndarray:
+    """C-contiguous complex128 copy of ``array``."""
+    return np.ascontiguousarray(np.asarray(array, dtype=np.complex128))
+
+
+def dptr(array: Optional[np.ndarray]):
+    """``double*`` view of a complex128 (or float64) array, or NULL."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(_DP)
+
+
+def iptr(array: np.ndarray):
+    """``int*`` view of an int32 array."""
+    return array.ctypes.data_as(_IP)
+
+
+def parse_backend_config(config: Optional[dict]) -> dict:
+    """Validate a backend configuration dictionary; only the MI355X engine
+    exists in this package."""
+    from oqupy_amd.config import DEFAULT_CHI_CAP
+    cfg = {} if config is None else dict(config)
+    backend = cfg.get("backend", "mi355x")
+    if backend != "mi355x":
+        raise ValueError(f"Unknown backend '{backend}'; this package provides "
+                         "only the 'mi355x' engine.")
+    precision = cfg.get("precision", "c128")
+    if precision != "c128":
+        raise ValueError(f"Unsupported precision '{precision}'.")
+    chi_cap = int(cfg.get("chi_cap", DEFAULT_CHI_CAP))
+    if chi_cap < 1:
+        raise ValueError("chi_cap must be positive.")
+    return {"backend": backend, "precision": precision, "chi_cap": chi_cap}
+
+
+def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192):
+    """Truncated SVD ``matrix = U diag(S) Vh`` by the engine's kernels, with
+    the relative truncation rule of ``split_node_full_svd``
+    (oqupy/backends/node_array.py:262). Returns ``(U, S, SVh, sweeps)`` with
+    ``SVh = diag(S) Vh``."""
+    require_gpu()
+    lib = load_library()
+    mat = as_c128(matrix)
+    m, n = mat.shape
+    r = min(m, n)
+    u = np.empty(m * r, dtype=np.complex128)
+    s = np.empty(r, dtype=np.float64)
+    svh = np.empty(r * n, dtype=np.complex128)
+    k = ctypes.c_int()
+    sweeps = ctypes.c_int()
+    check(lib.pte_split_svd(m, n, dptr(mat), float(epsrel), int(chi_cap),
+                            dptr(u), s.ctypes.data_as(_DP), dptr(svh),
+                            ctypes.byref(k), ctypes.byref(sweeps)))
+    kk = k.value
+    return (u[:m * kk].reshape(m, kk), s[:kk].copy(),
+            svh[:kk * n].reshape(kk, n), sweeps.value)

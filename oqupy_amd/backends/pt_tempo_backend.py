@@ -1,0 +1,172 @@
+"""PT-TEMPO backend running on the MI355X engine.
+
+Keeps the interface of the reference's ``PtTempoBackend``
+(oqupy/backends/pt_tempo_backend.py:28-316): ``initialize``, ``compute_step``,
+``step``, ``num_steps``, ``get_mpo_tensor`` and ``update_process_tensor``.
+The influence callable is evaluated here, on the host, and its arrays are
+handed to the engine: for dk = 0 .. min(N, dkmax+1)-1 before initialisation,
+and influence(-step) at every growth step (a rectangle influence when a
+correlation time is added, else ``None``), exactly where the reference calls
+it. The tensor chain stays in device memory.
+"""
+
+import ctypes
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+from numpy import ndarray
+
+from oqupy_amd.backends import engine as eng
+
+
+class PtTempoBackend:
+    """Process-tensor TEMPO on the MI355X engine."""
+
+    def __init__(self, dimension: int, influence: Callable[[int], ndarray],
+                 process_tensor, sum_north: ndarray, sum_west: ndarray,
+                 num_steps: int, dkmax: int, epsrel: float, config: Dict,
+                 degeneracy_maps: Optional[List[ndarray]] = None) -> None:
+        self._dimension = dimension
+        self._influence = influence
+        self._process_tensor = process_tensor
+        self._sum_north = np.asarray(sum_north)
+        self._sum_west = np.asarray(sum_west)
+        self._num_steps = num_steps
+        self._dkmax = dkmax
+        self._epsrel = epsrel
+        self._config = eng.parse_backend_config(config)
+        self._degeneracy_maps = degeneracy_maps
+        self._num_infl = min(num_steps, dkmax + 1)
+        self._step = None
+        self._handle = None
+        self.step_stats = []
+        self._lib = eng.load_library()
+
+    @property
+    def step(self) -> Optional[int]:
+        """The current step (None before ``initialize``)."""
+        return self._step
+
+    @property
+    def num_steps(self) -> int:
+        """Number of steps of the process tensor."""
+        return self._num_steps
+
+    @property
+    def handle(self):
+        """Opaque engine handle (valid after ``initialize``)."""
+        return self._handle
+
+    def _create(self) -> None:
+        eng.require_gpu()
+        cfg = eng._PtConfig(dimension=self._dimension,
+                            num_steps=self._num_steps, dkmax=self._dkmax,
+                            epsrel=self._epsrel, precision=0,
+                            chi_cap=self._config["chi_cap"],
+                            n_north=int(self._sum_north.size))
+        handle = ctypes.c_void_p()
+        eng.check(self._lib.pte_pt_create(ctypes.byref(cfg),
+                                          ctypes.byref(handle)))
+        self._handle = handle
+
+    def initialize(self) -> None:
+        """Build the MPO and MPS and sweep them (pt_tempo_backend.py:105-190)."""
+        self._create()
+        for dk in range(self._num_infl):
+            infl = eng.as_c128(self._influence(dk))
+            rows = infl.shape[0]
+            cols = infl.shape[1] if infl.ndim == 2 else 0
+            eng.check(self._lib.pte_pt_set_influence(
+                self._handle, dk, eng.dptr(infl), rows, cols))
+        if self._degeneracy_maps is not None:
+            nmap = np.ascontiguousarray(self._degeneracy_maps[0],
+                                        dtype=np.int32)
+            wmap = np.ascontiguousarray(self._degeneracy_maps[1],
+                                        dtype=np.int32)
+            eng.check(self._lib.pte_pt_set_degeneracy_maps(
+                self._handle, eng.iptr(nmap), eng.iptr(wmap), int(nmap.size)))
+        eng.check(self._lib.pte_pt_initialize(self._handle))
+        self._step = 1
+
+    def compute_step(self) -> bool:
+        """One PT-TEMPO step (pt_tempo_backend.py:192-282); returns whether
+        more steps follow."""
+        next_step = self._step + 1
+        end_phase = next_step > self._num_steps - self._num_infl + 1
+        last = None
+        rows = cols = 0
+        if not end_phase:
+            infl = self._influence(int(0 - next_step))
+            if infl is not None:
+                last = eng.as_c128(infl)
+                rows, cols = last.shape
+        more = ctypes.c_int()
+        eng.check(self._lib.pte_pt_step(self._handle, eng.dptr(last), rows,
+                                        cols, ctypes.byref(more)))
+        self._step = next_step
+        stats = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(),
+                 ctypes.c_double())
+        eng.check(self._lib.pte_pt_last_step_stats(
+            self._handle, *[ctypes.byref(s) for s in stats]))
+        self.step_stats.append(tuple(s.value for s in stats))
+        return bool(more.value)
+
+    def bond_dimensions(self) -> List[int]:
+        """Bond dimensions of the device-resident MPS."""
+        n = ctypes.c_int()
+        eng.check(self._lib.pte_pt_num_sites(self._handle, ctypes.byref(n)))
+        out = np.zeros(max(n.value - 1, 0), dtype=np.int32)
+        eng.check(self._lib.pte_pt_bond_dims(self._handle, eng.iptr(out),
+                                             int(out.size)))
+        return [int(b) for b in out]
+
+    def get_mpo_tensor(self, step: int) -> ndarray:
+        """PT-MPO tensor ``[bond_past, bond_future, d^2]`` of ``step``
+        (pt_tempo_backend.py:284-307)."""
+        assert self._step is not None and self._step >= self._num_steps
+        assert step < self._num_steps
+        bonds = self.bond_dimensions()
+        left = 1 if step == 0 else bonds[step - 1]
+        right = 1 if step == self._num_steps - 1 else bonds[step]
+        count = left * right * self._dimension ** 2
+        out = np.empty(count, dtype=np.complex128)
+        shape = np.zeros(3, dtype=np.int32)
+        eng.check(self._lib.pte_pt_get_mpo_tensor(
+            self._handle, step, eng.dptr(out), count, eng.iptr(shape)))
+        return out.reshape(tuple(int(s) for s in shape))
+
+    def compute_caps(self) -> List[ndarray]:
+        """Caps of the finished process tensor, computed on the device
+        (process_tensor.py:380-406)."""
+        eng.check(self._lib.pte_pt_compute_caps(self._handle))
+        caps = []
+        bonds = [1] + self.bond_dimensions() + [1]
+        for k in range(self._num_steps + 1):
+            out = np.empty(bonds[k], dtype=np.complex128)
+            length = ctypes.c_int()
+            eng.check(self._lib.pte_pt_get_cap(
+                self._handle, k, eng.dptr(out), bonds[k],
+                ctypes.byref(length)))
+            caps.append(out[:length.value].copy())
+        return caps
+
+    def update_process_tensor(self) -> None:
+        """Export every PT-MPO tensor and the caps into the process tensor
+        (pt_tempo_backend.py:309-316)."""
+        assert self._step >= self._num_steps
+        for step in reversed(range(self._num_steps)):
+            self._process_tensor.set_mpo_tensor(step,
+                                                self.get_mpo_tensor(step))
+        caps = self.compute_caps()
+        for k, cap in enumerate(caps):
+            self._process_tensor.set_cap_tensor(k, cap)
+        self._process_tensor.attach_engine(self)
+
+    def __del__(self) -> None:
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value:
+            try:
+                self._lib.pte_pt_destroy(handle)
+            except Exception:  # interpreter shutdown
+                pass
+            self._handle = None

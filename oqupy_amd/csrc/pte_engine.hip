@@ -1,0 +1,1466 @@
+// Host side of the MI355X process-tensor engine: device-resident tensor chains,
+// the PT-TEMPO and TEMPO step logic, compute_dynamics, and the C ABI declared
+// in include/ptengine.h.
+//
+// The step logic restates the reference backends
+// (oqupy/backends/pt_tempo_backend.py, oqupy/backends/tempo_backend.py,
+// oqupy/backends/node_array.py) on explicit [left, physical, right] site
+// tensors and [west, in, out, east] influence nodes; every contraction,
+// factorisation and truncation runs in the kernels of pte_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ptengine.h"
+#include "pte_device.h"
+#include "pte_kernels.h"
+
+namespace pte {
+
+using cplx = std::complex<double>;
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(x)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (x);                                                     \
+    if (e_ != hipSuccess)                                                    \
+      throw Error(PTE_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static void require(bool ok, int code, const std::string& msg) {
+  if (!ok) throw Error(code, msg);
+}
+
+static int grid_for(long total, int per_block = 256) {
+  long g = (total + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 65535L * 8) g = 65535L * 8;
+  return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  double2* p = nullptr;
+  size_t cap = 0;  // complex entries
+  void ensure(size_t n, hipStream_t s) {
+    if (n <= cap) return;
+    if (p) HIP_CHECK(hipFreeAsync(p, s));
+    size_t c = std::max<size_t>(n, cap * 3 / 2);
+    c = std::max<size_t>(c, 256);
+    HIP_CHECK(hipMallocAsync((void**)&p, c * sizeof(double2), s));
+    cap = c;
+  }
+  void release(hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// A tensor on the device with up to four dimensions (row-major).
+struct DTensor {
+  double2* p = nullptr;
+  int d[4] = {1, 1, 1, 1};
+  long size() const { return (long)d[0] * d[1] * d[2] * d[3]; }
+};
+
+static DTensor dalloc(int a, int b, int c, int e, hipStream_t s) {
+  DTensor t;
+  t.d[0] = a; t.d[1] = b; t.d[2] = c; t.d[3] = e;
+  long n = std::max<long>(1, t.size());
+  HIP_CHECK(hipMallocAsync((void**)&t.p, n * sizeof(double2), s));
+  return t;
+}
+static void dfree(DTensor& t, hipStream_t s) {
+  if (t.p) (void)hipFreeAsync(t.p, s);
+  t.p = nullptr;
+}
+static DTensor dupload(const std::vector<cplx>& h, int a, int b, int c, int e,
+                       hipStream_t s) {
+  DTensor t = dalloc(a, b, c, e, s);
+  require((long)h.size() == t.size(), PTE_ERR_ARG, "internal: upload size mismatch");
+  HIP_CHECK(hipMemcpyAsync(t.p, h.data(), h.size() * sizeof(double2),
+                           hipMemcpyHostToDevice, s));
+  return t;
+}
+
+// An influence node [west][in][out][east]; middle nodes of the delta form
+// W[w,p,s,v] = delta(w,v) delta(p,s) I[w,p] keep only I (west-major).
+struct Node {
+  DTensor dense;   // [wl][n][s][wr] (valid when kind == NODE_DENSE)
+  DTensor delta;   // [wl][p] weights (valid when kind == NODE_DELTA)
+  int kind = NODE_DENSE;
+  int wl = 1, n = 1, s = 1, wr = 1;
+};
+
+static Node make_dense(const std::vector<cplx>& h, int wl, int n, int s, int wr,
+                       hipStream_t st) {
+  Node nd;
+  nd.kind = NODE_DENSE;
+  nd.wl = wl; nd.n = n; nd.s = s; nd.wr = wr;
+  nd.dense = dupload(h, wl, n, s, wr, st);
+  return nd;
+}
+// I is given as [w][p] (w: the bond index carried along the chain)
+static Node make_delta(const std::vector<cplx>& Iwp, int w, int p, hipStream_t st) {
+  Node nd;
+  nd.kind = NODE_DELTA;
+  nd.wl = w; nd.n = p; nd.s = p; nd.wr = w;
+  nd.delta = dupload(Iwp, w, p, 1, 1, st);
+  return nd;
+}
+static void free_node(Node& nd, hipStream_t s) {
+  dfree(nd.dense, s);
+  dfree(nd.delta, s);
+}
+// dense host copy of a node (for closing legs on the host)
+static std::vector<cplx> node_dense_host(const Node& nd, hipStream_t s) {
+  std::vector<cplx> out((size_t)nd.wl * nd.n * nd.s * nd.wr, cplx(0.0, 0.0));
+  if (nd.kind == NODE_DENSE) {
+    HIP_CHECK(hipMemcpyAsync(out.data(), nd.dense.p, out.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<cplx> I((size_t)nd.wl * nd.n);
+    HIP_CHECK(hipMemcpyAsync(I.data(), nd.delta.p, I.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int w = 0; w < nd.wl; ++w)
+      for (int p = 0; p < nd.n; ++p)
+        out[(((size_t)w * nd.n + p) * nd.s + p) * nd.wr + w] = I[(size_t)w * nd.n + p];
+  }
+  return out;
+}
+
+static std::vector<cplx> to_cplx(const double* data, long count) {
+  std::vector<cplx> v(count);
+  for (long i = 0; i < count; ++i) {
+    v[i] = cplx(data[2 * i], data[2 * i + 1]);
+    require(std::isfinite(data[2 * i]) && std::isfinite(data[2 * i + 1]),
+            PTE_ERR_NUMERIC, "non-finite value in an input tensor");
+  }
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// workspace and the truncated SVD
+// ---------------------------------------------------------------------------
+struct Workspace {
+  hipStream_t stream = nullptr;
+  DevBuf G, X, V, carry, carry2, svh;
+  double* sigma = nullptr;
+  double* sorted = nullptr;
+  int* perm = nullptr;
+  size_t nsv_cap = 0;
+  unsigned* bar = nullptr;
+  int* flags = nullptr;
+  double* fro2 = nullptr;
+  SvdInfo* info_d = nullptr;
+  SvdInfo* info_h = nullptr;
+  int max_sweeps = 80;
+  int max_coop_blocks = 0;
+  double epsrel = 1e-7;
+  int chi_cap = 512;
+  // statistics of the current step
+  int n_svd = 0, sweeps = 0, max_bond = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  void init(double eps, int cap) {
+    epsrel = eps;
+    chi_cap = cap;
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    hipMemPool_t pool;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = ~0ull;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    HIP_CHECK(hipMalloc((void**)&bar, sizeof(unsigned)));
+    HIP_CHECK(hipMalloc((void**)&flags, sizeof(int) * max_sweeps));
+    HIP_CHECK(hipMalloc((void**)&fro2, sizeof(double)));
+    HIP_CHECK(hipMalloc((void**)&info_d, sizeof(SvdInfo)));
+    HIP_CHECK(hipHostMalloc((void**)&info_h, sizeof(SvdInfo), hipHostMallocDefault));
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(jacobi_kernel), 256, 0));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    max_coop_blocks = per_cu * prop.multiProcessorCount;
+    require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+  }
+
+  void destroy() {
+    if (!stream) return;
+    G.release(stream); X.release(stream); V.release(stream);
+    carry.release(stream); carry2.release(stream); svh.release(stream);
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
+    (void)hipFree(bar); (void)hipFree(flags); (void)hipFree(fro2); (void)hipFree(info_d);
+    (void)hipHostFree(info_h);
+    (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
+    (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  void ensure_sv(size_t n) {
+    if (n <= nsv_cap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
+    size_t c = std::max<size_t>(n, 1024);
+    HIP_CHECK(hipMalloc((void**)&sigma, c * sizeof(double)));
+    HIP_CHECK(hipMalloc((void**)&sorted, c * sizeof(double)));
+    HIP_CHECK(hipMalloc((void**)&perm, c * sizeof(int)));
+    nsv_cap = c;
+  }
+
+  // Jacobi SVD of X (m x n, column-major, ld = m, already in X) with V (n x n)
+  // accumulated; sorts and truncates. Returns the kept rank.
+  int svd(int m, int n) {
+    require(m >= 1 && n >= 1, PTE_ERR_ARG, "internal: empty SVD");
+    V.ensure((size_t)n * n, stream);
+    ensure_sv((size_t)n);
+    int nb = (n + kJacobiBlock - 1) / kJacobiBlock;
+    if (nb % 2) ++nb;
+    if (nb < 2) nb = 2;
+    const int grid = nb / 2;
+    require(grid <= max_coop_blocks, PTE_ERR_CAPACITY,
+            "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned), stream));
+    HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, stream));
+    HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), stream));
+    JacobiArgs ja;
+    ja.X = X.p; ja.m = m; ja.n = n; ja.ldx = m;
+    ja.V = V.p; ja.ldv = n;
+    ja.nb = nb;
+    ja.tol = std::max(m, 16) * 2.220446049250313e-16;
+    ja.abs_tol_scale = 2.220446049250313e-16;
+    ja.max_sweeps = max_sweeps;
+    ja.bar = bar; ja.flags = flags; ja.fro2 = fro2;
+    ja.info = &info_d->sweeps;
+    void* kargs[] = {&ja};
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
+                                         dim3(grid), dim3(256), kargs, 0, stream));
+    column_norm_kernel<<<n, 256, 0, stream>>>(X.p, m, n, m, sigma);
+    rank_sort_kernel<<<(n + 255) / 256, 256, 0, stream>>>(sigma, n, perm, sorted);
+    truncate_kernel<<<1, 1024, 0, stream>>>(sorted, n, epsrel, info_d, chi_cap);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
+                             stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    require(info_h->sweeps > 0, PTE_ERR_NUMERIC,
+            "Jacobi SVD did not converge (" + std::to_string(m) + "x" +
+                std::to_string(n) + ")");
+    require(!info_h->overflow, PTE_ERR_CAPACITY,
+            "bond dimension " + std::to_string(info_h->rank) +
+                " exceeds the capacity chi_cap=" + std::to_string(chi_cap));
+    require(info_h->rank >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
+    n_svd += 1;
+    sweeps += info_h->sweeps;
+    max_bond = std::max(max_bond, info_h->rank);
+    return info_h->rank;
+  }
+
+  void emit(const double2* src, long lds, int A, int B, long dj0, long djk,
+            long da0, long dak, long db0, long dbk, int conj, double2* dst, int k) {
+    EmitArgs ea;
+    ea.src = src; ea.lds = lds; ea.perm = perm; ea.info = info_d;
+    ea.A = A; ea.B = B;
+    ea.dj0 = dj0; ea.djk = djk; ea.da0 = da0; ea.dak = dak; ea.db0 = db0; ea.dbk = dbk;
+    ea.conj = conj; ea.dst = dst;
+    emit_kernel<<<grid_for((long)A * B * k), 256, 0, stream>>>(ea);
+  }
+
+  void gemm(int M, int N, int K, const double2* A, long sai, long sak,
+            const double2* B, long sbk, long sbj, double2* C, long sci, long scj) {
+    GemmArgs g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.sai = sai; g.sak = sak;
+    g.B = B; g.sbk = sbk; g.sbj = sbj;
+    g.C = C; g.sci = sci; g.scj = scj;
+    g.conjA = 0; g.conjB = 0;
+    g.alpha = make_double2(1.0, 0.0);
+    g.beta_zero = 1;
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    zgemm_kernel<<<grid, 256, 0, stream>>>(g);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// chain operations (node_array.py) on device sites [l][p][r]
+// ---------------------------------------------------------------------------
+struct Chain {
+  Workspace* ws = nullptr;
+  std::vector<DTensor> sites;  // d[0]=left, d[1]=phys, d[2]=right
+
+  hipStream_t st() const { return ws->stream; }
+
+  void set_site(int i, DTensor t) {
+    dfree(sites[i], st());
+    sites[i] = t;
+  }
+
+  // Truncating SVD sweep left to right from `start` to `stop` (exclusive of
+  // the SVD at `stop`) -- NodeArray.svd_sweep, node_array.py:251-273.
+  void sweep_right(int start, int stop) {
+    for (int i = start; i < stop; ++i) {
+      DTensor& A = sites[i];
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      // M = A as (Dl*P) x Dr; Jacobi on X = M^H (Dr x Dl*P), X[r + c*Dr] = conj(A[c*Dr + r])
+      const long cnt = (long)Dl * P * Dr;
+      ws->X.ensure(cnt, st());
+      conj_copy_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, cnt, ws->X.p);
+      const int k = ws->svd(Dr, Dl * P);
+      // new site [l][p][j] = V[(l*P+p) + perm[j]*ldv]
+      DTensor U = dalloc(Dl, P, k, 1, st());
+      ws->emit(ws->V.p, Dl * P, Dl * P, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+      // svh[j][r] = conj(Z[r + perm[j]*Dr])
+      ws->svh.ensure((size_t)k * Dr, st());
+      ws->emit(ws->X.p, Dr, Dr, 1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
+      // next site: [j][q][s] = sum_r svh[j][r] next[r][q][s]
+      DTensor& Bn = sites[i + 1];
+      require(Bn.d[0] == Dr, PTE_ERR_ARG, "internal: bond mismatch in sweep");
+      const int Pq = Bn.d[1], Ds = Bn.d[2];
+      DTensor Nn = dalloc(k, Pq, Ds, 1, st());
+      ws->gemm(k, Pq * Ds, Dr, ws->svh.p, Dr, 1, Bn.p, (long)Pq * Ds, 1, Nn.p,
+               (long)Pq * Ds, 1);
+      set_site(i, U);
+      set_site(i + 1, Nn);
+    }
+  }
+
+  // Truncating SVD sweep right to left from `start` down to `stop`
+  // (exclusive) -- NodeArray.svd_sweep, node_array.py:274-296.
+  void sweep_left(int start, int stop) {
+    for (int i = start; i > stop; --i) {
+      DTensor& A = sites[i];
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      // M[(p,r), l] = A[l][p][r]; X = M^H (Dl x P*Dr): X[l + c*Dl] = conj(A[l*(P*Dr) + c])
+      ws->X.ensure((long)Dl * P * Dr, st());
+      conj_transpose_kernel<<<grid_for((long)Dl * P * Dr), 256, 0, st()>>>(
+          A.p, Dl, P * Dr, ws->X.p);
+      const int k = ws->svd(Dl, P * Dr);
+      // new site [j][p][r] = V[(p*Dr + r) + perm[j]*ldv]
+      DTensor U = dalloc(k, P, Dr, 1, st());
+      ws->emit(ws->V.p, (long)P * Dr, P * Dr, 1, /*dj*/ (long)P * Dr, 0, /*da*/ 1, 0, 0,
+               0, 0, U.p, k);
+      // svh[j][l] = conj(Z[l + perm[j]*Dl])
+      ws->svh.ensure((size_t)k * Dl, st());
+      ws->emit(ws->X.p, Dl, Dl, 1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
+      // previous site: [l'][q][j] = sum_l prev[l'][q][l] svh[j][l]
+      DTensor& Bp = sites[i - 1];
+      require(Bp.d[2] == Dl, PTE_ERR_ARG, "internal: bond mismatch in sweep");
+      const int Dl2 = Bp.d[0], Pq = Bp.d[1];
+      DTensor Np = dalloc(Dl2, Pq, k, 1, st());
+      ws->gemm(Dl2 * Pq, k, Dl, Bp.p, Dl, 1, ws->svh.p, 1, Dl, Np.p, k, 1);
+      set_site(i, U);
+      set_site(i - 1, Np);
+    }
+  }
+
+  // Zip `mpo` into the last mpo.size() sites, right to left, with a truncated
+  // SVD at every site but the leftmost -- NodeArray.zip_up, node_array.py:413-554
+  // with direction 'left'.
+  void zip_left(std::vector<Node>& mpo) {
+    const int n = (int)mpo.size();
+    const int first = (int)sites.size() - n;
+    require(first >= 0, PTE_ERR_ARG, "internal: MPO longer than the MPS");
+    // identity carry over (right bond of the last site, east leg of the last node)
+    int Dx, Cr, Cv;  // carry dims: [r][x][v]
+    {
+      const DTensor& A = sites[first + n - 1];
+      const Node& W = mpo[n - 1];
+      Cr = A.d[2]; Cv = W.wr; Dx = Cr * Cv;
+      ws->carry.ensure((size_t)Cr * Dx * Cv, st());
+      identity_carry_kernel<<<grid_for((long)Cr * Dx * Cv), 256, 0, st()>>>(
+          ws->carry.p, Cr, Cv);
+    }
+    for (int j = n - 1; j >= 0; --j) {
+      const int ia = first + j;
+      DTensor& A = sites[ia];
+      Node& W = mpo[j];
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      require(Dr == Cr && W.wr == Cv && W.n == P, PTE_ERR_ARG,
+              "internal: leg mismatch in zip-up");
+      const int Wl = W.wl, S = W.s;
+      // G[(l,p),(x,v)] = sum_r A[(l,p), r] carry[r, (x,v)]
+      ws->G.ensure((size_t)Dl * P * Dx * Cv, st());
+      ws->gemm(Dl * P, Dx * Cv, Dr, A.p, Dr, 1, ws->carry.p, (long)Dx * Cv, 1, ws->G.p,
+               (long)Dx * Cv, 1);
+      MpoApplyArgs ma;
+      ma.G = ws->G.p;
+      ma.W = W.dense.p;
+      ma.I = W.delta.p;
+      ma.kind = W.kind;
+      ma.Dl = Dl; ma.P = P; ma.Dx = Dx; ma.V = Cv; ma.Wl = Wl; ma.S = S;
+      const long total = (long)Dx * S * Dl * Wl;
+      if (j == 0) {
+        require(Wl == 1, PTE_ERR_ARG, "internal: first node has a west leg");
+        DTensor T = dalloc(Dl, S, Dx, 1, st());
+        ma.mode = MPO_OUT_SITE;
+        ma.out = T.p;
+        ma.ldx = 0;
+        mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        set_site(ia, T);
+        break;
+      }
+      // X = M^H with M[(x,s),(l,w)]: X is (Dl*Wl) x (Dx*S)
+      const int mrows = Dl * Wl, ncols = Dx * S;
+      ws->X.ensure((size_t)mrows * ncols, st());
+      ma.mode = MPO_OUT_SVD_H;
+      ma.out = ws->X.p;
+      ma.ldx = mrows;
+      mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+      const int k = ws->svd(mrows, ncols);
+      // new site [j][s][x] = V[(x*S + s) + perm[j]*ldv]   (a = x, b = s)
+      DTensor U = dalloc(k, S, Dx, 1, st());
+      ws->emit(ws->V.p, ncols, Dx, S, /*dj*/ (long)S * Dx, 0, /*da*/ 1, 0, /*db*/ Dx, 0,
+               0, U.p, k);
+      // next carry [l][j][w] = conj(Z[(l*Wl + w) + perm[j]*mrows])   (a = l, b = w)
+      ws->carry2.ensure((size_t)Dl * k * Wl, st());
+      ws->emit(ws->X.p, mrows, Dl, Wl, /*dj*/ Wl, 0, /*da*/ 0, Wl, /*db*/ 1, 0, 1,
+               ws->carry2.p, k);
+      std::swap(ws->carry, ws->carry2);
+      Cr = Dl; Dx = k; Cv = Wl;
+      set_site(ia, U);
+    }
+  }
+
+  // Zip `mpo` (same length as the chain) left to right with a truncated SVD
+  // at every site but the rightmost -- NodeArray.zip_up with direction
+  // 'right', left_index=0, right_index=-1 (TEMPO, tempo_backend.py:539-547).
+  void zip_right(std::vector<Node>& mpo) {
+    const int n = (int)mpo.size();
+    require((int)sites.size() == n, PTE_ERR_ARG, "internal: zip lengths differ");
+    // carry [(x,w)][l] with x = (l0, w0): identity
+    int Dx, Cl, Cw;
+    {
+      const DTensor& A = sites[0];
+      const Node& W = mpo[0];
+      Cl = A.d[0]; Cw = W.wl; Dx = Cl * Cw;
+      // carry'[(x*Cw + w)*Cl + l] = delta(x, l*Cw + w): same pattern as the
+      // identity carry with (R, V) = (Cl, Cw) read as [(x,w)][l]
+      std::vector<cplx> h((size_t)Dx * Cw * Cl, cplx(0.0, 0.0));
+      for (int x = 0; x < Dx; ++x)
+        for (int w = 0; w < Cw; ++w)
+          for (int l = 0; l < Cl; ++l)
+            if (x == l * Cw + w) h[((size_t)x * Cw + w) * Cl + l] = 1.0;
+      ws->carry.ensure(h.size(), st());
+      HIP_CHECK(hipMemcpyAsync(ws->carry.p, h.data(), h.size() * sizeof(double2),
+                               hipMemcpyHostToDevice, st()));
+      HIP_CHECK(hipStreamSynchronize(st()));
+    }
+    for (int j = 0; j < n; ++j) {
+      DTensor& A = sites[j];
+      Node& W = mpo[j];
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      require(Dl == Cl && W.wl == Cw && W.n == P, PTE_ERR_ARG,
+              "internal: leg mismatch in zip-up (right)");
+      const int E = W.wr, S = W.s;
+      // G[(x,w),(p,r)] = sum_l carry[(x,w), l] A[l, (p,r)]
+      ws->G.ensure((size_t)Dx * Cw * P * Dr, st());
+      ws->gemm(Dx * Cw, P * Dr, Dl, ws->carry.p, Dl, 1, A.p, (long)P * Dr, 1, ws->G.p,
+               (long)P * Dr, 1);
+      MpoApplyArgs ma;
+      ma.G = ws->G.p;
+      ma.W = W.dense.p;
+      ma.I = W.delta.p;
+      ma.kind = W.kind;
+      ma.Dl = Dr; ma.P = P; ma.Dx = Dx; ma.V = Cw; ma.Wl = E; ma.S = S;
+      const long total = (long)Dx * S * Dr * E;
+      if (j == n - 1) {
+        require(Dr == 1, PTE_ERR_ARG, "internal: last site has a right bond");
+        DTensor T = dalloc(Dx, S, E, 1, st());
+        ma.mode = MPO_OUT_SITE;
+        ma.out = T.p;
+        ma.ldx = 0;
+        mpo_apply_right_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        set_site(j, T);
+        break;
+      }
+      // X = M^H with M[(x,s),(r,e)]: X is (Dr*E) x (Dx*S)
+      const int mrows = Dr * E, ncols = Dx * S;
+      ws->X.ensure((size_t)mrows * ncols, st());
+      ma.mode = MPO_OUT_SVD_H;
+      ma.out = ws->X.p;
+      ma.ldx = mrows;
+      mpo_apply_right_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+      const int k = ws->svd(mrows, ncols);
+      // new site [x][s][j] = V[(x*S+s) + perm[j]*ldv]  (a = x*S+s)
+      DTensor U = dalloc(Dx, S, k, 1, st());
+      ws->emit(ws->V.p, ncols, Dx * S, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+      // carry [(j,e)][r] = conj(Z[(r*E + e) + perm[j]*mrows])  (a = r, b = e)
+      ws->carry2.ensure((size_t)k * E * Dr, st());
+      ws->emit(ws->X.p, mrows, Dr, E, /*dj*/ (long)E * Dr, 0, /*da*/ 1, 0, /*db*/ Dr, 0, 1,
+               ws->carry2.p, k);
+      std::swap(ws->carry, ws->carry2);
+      Cl = Dr; Cw = E; Dx = k;
+      set_site(j, U);
+    }
+  }
+
+  void clear() {
+    for (auto& s : sites) dfree(s, st());
+    sites.clear();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// host construction helpers (oqupy/util.py create_delta layouts)
+// ---------------------------------------------------------------------------
+struct HostInfl {
+  std::vector<cplx> v;
+  int rows = 0, cols = 0;  // cols == 0: vector
+  bool set = false;
+};
+
+static cplx hget(const HostInfl& h, int i, int j) { return h.v[(size_t)i * h.cols + j]; }
+
+// ---------------------------------------------------------------------------
+// PT-TEMPO engine (oqupy/backends/pt_tempo_backend.py)
+// ---------------------------------------------------------------------------
+struct PtEngine {
+  pte_pt_config cfg;
+  Workspace ws;
+  Chain mps;
+  std::vector<Node> mpo;
+  std::vector<HostInfl> infl;
+  std::vector<int> north_map, west_map;
+  int num_infl = 0;
+  int step = 0;  // 0: not initialised
+  std::vector<cplx> sum_north_scaled;
+  // caps (device) after pte_pt_compute_caps
+  std::vector<DTensor> caps;
+  double last_ms = 0.0;
+
+  hipStream_t st() { return ws.stream; }
+
+  void init_engine() {
+    ws.init(cfg.epsrel, cfg.chi_cap);
+    mps.ws = &ws;
+    num_infl = std::min(cfg.num_steps, cfg.dkmax + 1);
+    infl.resize(num_infl);
+  }
+
+  // pt_tempo_backend.py:105-190
+  void initialize() {
+    const int d = cfg.dimension, d2 = d * d;
+    for (int i = 0; i < num_infl; ++i)
+      require(infl[i].set, PTE_ERR_STATE, "influence " + std::to_string(i) + " not set");
+    sum_north_scaled.assign(cfg.n_north, cplx((double)d, 0.0));
+    const bool uniq = !north_map.empty();
+    int n_north = 0, n_west = 0;
+    if (uniq) {
+      n_north = *std::max_element(north_map.begin(), north_map.end()) + 1;
+      n_west = *std::max_element(west_map.begin(), west_map.end()) + 1;
+    }
+    mps.sites.assign(num_infl, DTensor());
+    mpo.assign(num_infl, Node());
+    for (int i = 0; i < num_infl; ++i) {
+      const HostInfl& h = infl[i];
+      if (i == 0) {
+        if (uniq) {
+          require(h.cols == 0 && h.rows == n_north, PTE_ERR_ARG,
+                  "influence 0 must be a vector of the north classes");
+          std::vector<cplx> node((size_t)n_west * d2 * n_north, 0.0);
+          std::vector<cplx> site((size_t)d2 * n_north, 0.0);
+          for (int i1 = 0; i1 < d2; ++i1) {
+            const cplx val = h.v[north_map[i1]] / (double)d;
+            node[((size_t)west_map[i1] * d2 + i1) * n_north + north_map[i1]] = val;
+            site[(size_t)i1 * n_north + north_map[i1]] = val / (double)d;
+          }
+          mpo[0] = make_dense(node, 1, n_west, d2, n_north, st());
+          mps.sites[0] = dupload(site, 1, d2, n_north, 1, st());
+        } else {
+          require(h.rows == d2 && h.cols == d2, PTE_ERR_ARG, "influence 0 must be d^2 x d^2");
+          // create_delta(I0/d, [1,1,0]): W[j][j][i] = I0[i][j]/d
+          std::vector<cplx> node((size_t)d2 * d2 * d2, 0.0);
+          std::vector<cplx> site((size_t)d2 * d2, 0.0);
+          for (int a = 0; a < d2; ++a)
+            for (int b = 0; b < d2; ++b) {
+              const cplx val = hget(h, a, b) / (double)d;
+              node[((size_t)b * d2 + b) * d2 + a] = val;
+              site[(size_t)a * d2 + b] = hget(h, b, a) / (double)d / (double)d;  // (I0/d).T/d
+            }
+          mpo[0] = make_dense(node, 1, d2, d2, d2, st());
+          mps.sites[0] = dupload(site, 1, d2, d2, 1, st());
+        }
+        if (num_infl == 1) {
+          // a single node keeps its open right leg, nothing else to build
+        }
+      } else if (i == num_infl - 1) {
+        require(h.cols > 0, PTE_ERR_ARG, "influence must be a matrix");
+        // add_singleton(1), add_singleton(3): W[l][0][s][0] = I[l][s]
+        mpo[i] = make_dense(h.v, h.rows, 1, h.cols, 1, st());
+        mps.sites[i] = dupload(h.v, h.rows, h.cols, 1, 1, st());
+      } else {
+        require(h.cols > 0, PTE_ERR_ARG, "influence must be a matrix");
+        // create_delta(I, [0,1,1,0]): W[l][n][n][l] = I[l][n]
+        mpo[i] = make_delta(h.v, h.rows, h.cols, st());
+        // create_delta(I/d, [0,1,0]): A[l][p][l] = I[l][p]/d
+        std::vector<cplx> site((size_t)h.rows * h.cols * h.rows, 0.0);
+        for (int l = 0; l < h.rows; ++l)
+          for (int p = 0; p < h.cols; ++p)
+            site[((size_t)l * h.cols + p) * h.rows + l] = hget(h, l, p) / (double)d;
+        mps.sites[i] = dupload(site, h.rows, h.cols, h.rows, 1, st());
+      }
+    }
+    begin_stats();
+    const int n = (int)mps.sites.size();
+    mps.sweep_left(n - 1, 0);
+    mps.sweep_right(0, n - 1);
+    end_stats();
+    step = 1;
+  }
+
+  void begin_stats() {
+    ws.n_svd = 0; ws.sweeps = 0; ws.max_bond = 0;
+    HIP_CHECK(hipEventRecord(ws.ev0, st()));
+  }
+  void end_stats() {
+    HIP_CHECK(hipEventRecord(ws.ev1, st()));
+    HIP_CHECK(hipEventSynchronize(ws.ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+    last_ms = ms;
+  }
+
+  // pt_tempo_backend.py:192-282
+  bool compute_step(const double* last, int rows, int cols) {
+    require(step >= 1, PTE_ERR_STATE, "engine not initialised");
+    require(step < cfg.num_steps, PTE_ERR_STATE, "process tensor already complete");
+    step += 1;
+    const bool end_phase = step > cfg.num_steps - num_infl + 1;
+    begin_stats();
+    if (end_phase) {
+      require(mpo.size() >= 2, PTE_ERR_STATE, "internal: MPO too short to shrink");
+      // drop the last node, close the new last node's east leg with sum_north*d
+      free_node(mpo.back(), st());
+      mpo.pop_back();
+      Node& nd = mpo.back();
+      std::vector<cplx> dense = node_dense_host(nd, st());
+      require((int)sum_north_scaled.size() == nd.wr, PTE_ERR_ARG,
+              "sum_north length does not match the MPO bond");
+      std::vector<cplx> closed((size_t)nd.wl * nd.n * nd.s, 0.0);
+      for (size_t a = 0; a < closed.size(); ++a) {
+        cplx acc = 0.0;
+        for (int v = 0; v < nd.wr; ++v) acc += dense[a * nd.wr + v] * sum_north_scaled[v];
+        closed[a] = acc;
+      }
+      const int wl = nd.wl, nn = nd.n, ss = nd.s;
+      free_node(nd, st());
+      nd = make_dense(closed, wl, nn, ss, 1, st());
+    } else {
+      if (last != nullptr) {
+        std::vector<cplx> v = to_cplx(last, (long)rows * cols);
+        Node& nd = mpo.back();
+        require(rows == nd.wl && cols == nd.s, PTE_ERR_ARG,
+                "replacement influence has the wrong shape");
+        const int wl = nd.wl;
+        free_node(nd, st());
+        nd = make_dense(v, wl, 1, cols, 1, st());
+      }
+      // join the [[1.0]] site
+      std::vector<cplx> one(1, cplx(1.0, 0.0));
+      mps.sites.push_back(dupload(one, 1, 1, 1, 1, st()));
+    }
+    mps.zip_left(mpo);
+    mps.sweep_right(step - 2, (int)mps.sites.size() - 1);
+    end_stats();
+    return step < cfg.num_steps;
+  }
+
+  // pt_tempo_backend.py:284-307: [l][p][r] -> [l][r][p] * d
+  void get_mpo_tensor(int k, double* out, long cap, int* shape3) {
+    require(step >= cfg.num_steps, PTE_ERR_STATE, "process tensor not complete");
+    require((int)mps.sites.size() == cfg.num_steps, PTE_ERR_STATE,
+            "internal: MPS length differs from the number of steps");
+    require(k >= 0 && k < cfg.num_steps, PTE_ERR_ARG, "step index out of range");
+    const DTensor& s = mps.sites[k];
+    const int Dl = s.d[0], P = s.d[1], Dr = s.d[2];
+    shape3[0] = Dl; shape3[1] = Dr; shape3[2] = P;
+    require(cap >= (long)Dl * P * Dr, PTE_ERR_ARG, "output buffer too small");
+    std::vector<cplx> h((size_t)Dl * P * Dr);
+    HIP_CHECK(hipMemcpyAsync(h.data(), s.p, h.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, st()));
+    HIP_CHECK(hipStreamSynchronize(st()));
+    const double d = cfg.dimension;
+    for (int l = 0; l < Dl; ++l)
+      for (int r = 0; r < Dr; ++r)
+        for (int p = 0; p < P; ++p) {
+          const cplx v = h[((size_t)l * P + p) * Dr + r] * d;
+          const size_t o = ((size_t)l * Dr + r) * P + p;
+          out[2 * o] = v.real();
+          out[2 * o + 1] = v.imag();
+        }
+  }
+
+  // process_tensor.py:380-406, with trace_square = vec(1)/d; sites keep the
+  // [l][p][r] layout and the factor d of the export:
+  // cap_k[a] = sum_{p,b} d*A_k[a][p][b] cap_{k+1}[b] tr2[p]
+  void compute_caps() {
+    require(step >= cfg.num_steps, PTE_ERR_STATE, "process tensor not complete");
+    const int N = cfg.num_steps, d = cfg.dimension;
+    for (auto& c : caps) dfree(c, st());
+    caps.assign(N + 1, DTensor());
+    std::vector<cplx> one(1, cplx(1.0, 0.0));
+    caps[N] = dupload(one, 1, 1, 1, 1, st());
+    for (int k = N - 1; k >= 0; --k) {
+      const DTensor& s = mps.sites[k];
+      const int Dl = s.d[0], P = s.d[1], Dr = s.d[2];
+      require(P == d * d, PTE_ERR_ARG, "internal: PT site physical leg is not d^2");
+      // tmp[a][p] = sum_b A[a][p][b] cap[b]
+      DTensor tmp = dalloc(Dl, P, 1, 1, st());
+      ws.gemm(Dl * P, 1, Dr, s.p, Dr, 1, caps[k + 1].p, 1, 1, tmp.p, 1, 1);
+      // cap[a] = sum_p tmp[a][p] * d * tr2[p]   (d * tr2 = 1 on the diagonal)
+      std::vector<cplx> vec(P, 0.0);
+      for (int i = 0; i < d; ++i) vec[i * d + i] = 1.0;
+      DTensor dv = dupload(vec, P, 1, 1, 1, st());
+      DTensor c = dalloc(Dl, 1, 1, 1, st());
+      leg_vector_kernel<<<grid_for(Dl), 256, 0, st()>>>(tmp.p, Dl, P, 1, dv.p, c.p);
+      dfree(tmp, st());
+      dfree(dv, st());
+      caps[k] = c;
+    }
+    HIP_CHECK(hipStreamSynchronize(st()));
+  }
+
+  void destroy() {
+    for (auto& c : caps) dfree(c, st());
+    mps.clear();
+    for (auto& nd : mpo) free_node(nd, st());
+    mpo.clear();
+    ws.destroy();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// TEMPO engine (oqupy/backends/tempo_backend.py:314-626)
+// ---------------------------------------------------------------------------
+struct TempoEngine {
+  pte_tempo_config cfg;
+  Workspace ws;
+  Chain mps;
+  std::vector<HostInfl> infl;   // dk >= 0
+  std::vector<int> north_map, west_map;
+  std::vector<Node> mpo;        // reversed: [dk=K ... dk=0]
+  int step = -1;                // -1: not initialised
+  std::vector<cplx> sum_north, sum_west;
+  DTensor sum_north_d;
+
+  hipStream_t st() { return ws.stream; }
+
+  void init_engine() {
+    ws.init(cfg.epsrel, cfg.chi_cap);
+    mps.ws = &ws;
+  }
+
+  HostInfl& slot(int dk) {
+    if ((int)infl.size() <= dk) infl.resize(dk + 1);
+    return infl[dk];
+  }
+
+  // create_delta(I, [1,0,0,1]): W[w=j][n=i][s=i][e=j] = I[i][j] -> delta form
+  // with weights Iw[w][p] = I[p][w]
+  Node four_legs(const HostInfl& h) {
+    std::vector<cplx> Iw((size_t)h.cols * h.rows);
+    for (int i = 0; i < h.rows; ++i)
+      for (int j = 0; j < h.cols; ++j) Iw[(size_t)j * h.rows + i] = hget(h, i, j);
+    Node nd = make_delta(Iw, h.cols, h.rows, st());
+    return nd;
+  }
+
+  void initialize(const double* rho0, const double* unitary) {
+    const int d = cfg.dimension, d2 = d * d;
+    std::vector<cplx> U = to_cplx(unitary, (long)d * d);
+    // super_u = kron(U, conj(U)), super_u_dagg = kron(U^H, U^T)
+    std::vector<cplx> su((size_t)d2 * d2), sud((size_t)d2 * d2);
+    for (int a = 0; a < d; ++a)
+      for (int b = 0; b < d; ++b)
+        for (int c = 0; c < d; ++c)
+          for (int e = 0; e < d; ++e) {
+            const size_t r = (size_t)a * d + c, col = (size_t)b * d + e;
+            su[r * d2 + col] = U[a * d + b] * std::conj(U[c * d + e]);
+            sud[r * d2 + col] = std::conj(U[b * d + a]) * U[e * d + c];
+          }
+    sum_north.assign(cfg.n_north, cplx(1.0, 0.0));
+    sum_west.assign(cfg.n_west, cplx(1.0, 0.0));
+    sum_north_d = dupload(sum_north, cfg.n_north, 1, 1, 1, st());
+    const int count = (cfg.dkmax < 0) ? 1 : cfg.dkmax + 1;
+    std::vector<Node> nodes;
+    for (int i = 0; i < count; ++i) {
+      require((int)infl.size() > i && infl[i].set, PTE_ERR_STATE,
+              "influence " + std::to_string(i) + " not set");
+      const HostInfl& h = infl[i];
+      if (i == 0) {
+        if (!north_map.empty()) {
+          const int n_north = *std::max_element(north_map.begin(), north_map.end()) + 1;
+          const int n_west = *std::max_element(west_map.begin(), west_map.end()) + 1;
+          require(h.cols == 0 && h.rows == n_north, PTE_ERR_ARG,
+                  "influence 0 must be a vector of the north classes");
+          std::vector<cplx> node((size_t)n_west * d2 * n_north * d2, 0.0);
+          for (int i1 = 0; i1 < d2; ++i1)
+            node[(((size_t)west_map[i1] * d2 + i1) * n_north + north_map[i1]) * d2 + i1] =
+                h.v[north_map[i1]];
+          nodes.push_back(make_dense(node, n_west, d2, n_north, d2, st()));
+        } else {
+          require(h.rows == d2 && h.cols == d2, PTE_ERR_ARG, "influence 0 must be d^2 x d^2");
+          // W0[w][n][s][e] = delta(w,n,s,e) I0[n][n], then the basis change
+          // W'[w][m][s][f] = sum_{n,e} W0[w][n][s][e] sud[n][m] su[f][e]
+          std::vector<cplx> node((size_t)d2 * d2 * d2 * d2, 0.0);
+          for (int w = 0; w < d2; ++w)
+            for (int m = 0; m < d2; ++m)
+              for (int s = 0; s < d2; ++s)
+                for (int f = 0; f < d2; ++f) {
+                  // only n = e = w = s contributes
+                  if (w != s) continue;
+                  const cplx val = hget(h, w, w) * sud[(size_t)w * d2 + m] *
+                                   su[(size_t)f * d2 + w];
+                  node[(((size_t)w * d2 + m) * d2 + s) * d2 + f] = val;
+                }
+          nodes.push_back(make_dense(node, d2, d2, d2, d2, st()));
+        }
+      } else {
+        nodes.push_back(four_legs(h));
+      }
+    }
+    mpo.assign(nodes.rbegin(), nodes.rend());
+    std::vector<cplx> r0 = to_cplx(rho0, d2);
+    mps.clear();
+    mps.sites.push_back(dupload(r0, 1, d2, 1, 1, st()));
+    step = 0;
+  }
+
+  // a node whose west leg is closed with sum_west (apply_vector, left=True)
+  Node close_west(const Node& nd) {
+    std::vector<cplx> dense = node_dense_host(nd, st());
+    require((int)sum_west.size() == nd.wl, PTE_ERR_ARG,
+            "sum_west length does not match the MPO bond");
+    std::vector<cplx> closed((size_t)nd.n * nd.s * nd.wr, 0.0);
+    for (int w = 0; w < nd.wl; ++w)
+      for (size_t a = 0; a < closed.size(); ++a)
+        closed[a] += dense[(size_t)w * closed.size() + a] * sum_west[w];
+    return make_dense(closed, 1, nd.n, nd.s, nd.wr, st());
+  }
+
+  // tempo_backend.py:439-575
+  void compute_step(const double* p1, const double* p2, const double* extra, int rows,
+                    int cols, double* state_out) {
+    require(step >= 0, PTE_ERR_STATE, "engine not initialised");
+    step += 1;
+    const int d = cfg.dimension, d2 = d * d;
+    std::vector<cplx> P1 = to_cplx(p1, (long)d2 * d2), P2 = to_cplx(p2, (long)d2 * d2);
+    // window of the MPO (borrowed nodes except the replaced/closed ones)
+    std::vector<Node> window;
+    std::vector<Node> owned;
+    const int K = cfg.dkmax;
+    if (K < 0) {
+      window = mpo;
+      // grow the MPO with influence(len(window)) for the next step
+      const int dk = (int)window.size();
+      require((int)infl.size() > dk && infl[dk].set, PTE_ERR_STATE,
+              "influence " + std::to_string(dk) + " not set");
+      mpo.insert(mpo.begin(), four_legs(infl[dk]));
+    } else if (step <= K) {
+      window.assign(mpo.end() - step, mpo.end());
+    } else {
+      window = mpo;
+      if (extra != nullptr) {
+        HostInfl h;
+        h.v = to_cplx(extra, (long)rows * cols);
+        h.rows = rows; h.cols = cols; h.set = true;
+        owned.push_back(four_legs(h));
+        window[0] = owned.back();
+      }
+    }
+    owned.push_back(close_west(window[0]));
+    window[0] = owned.back();
+    // first half-step on the newest site: A[l][q][r] = sum_p P1[q][p] A[l][p][r]
+    {
+      DTensor& A = mps.sites.back();
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      require(P == d2, PTE_ERR_ARG, "internal: newest site is not d^2");
+      DTensor Pd = dupload(P1, d2, d2, 1, 1, st());
+      DTensor N = dalloc(Dl, P, Dr, 1, st());
+      for (int l = 0; l < Dl; ++l)
+        ws.gemm(P, Dr, P, Pd.p, P, 1, A.p + (long)l * P * Dr, Dr, 1,
+                N.p + (long)l * P * Dr, Dr, 1);
+      dfree(Pd, st());
+      mps.set_site((int)mps.sites.size() - 1, N);
+    }
+    // trace out the oldest site once the window is full
+    if (mps.sites.size() != window.size()) {
+      DTensor& A0 = mps.sites[0];
+      const int Dl = A0.d[0], P = A0.d[1], Dr = A0.d[2];
+      require(Dl == 1 && P == cfg.n_north, PTE_ERR_ARG, "internal: oldest site shape");
+      DTensor tr = dalloc(1, Dr, 1, 1, st());
+      leg_vector_kernel<<<grid_for(Dr), 256, 0, st()>>>(A0.p, 1, P, Dr, sum_north_d.p,
+                                                        tr.p);
+      DTensor& A1 = mps.sites[1];
+      const int P1d = A1.d[1], Dr1 = A1.d[2];
+      DTensor N = dalloc(1, P1d, Dr1, 1, st());
+      ws.gemm(1, P1d * Dr1, Dr, tr.p, Dr, 1, A1.p, (long)P1d * Dr1, 1, N.p,
+              (long)P1d * Dr1, 1);
+      dfree(tr, st());
+      mps.set_site(1, N);
+      dfree(mps.sites[0], st());
+      mps.sites.erase(mps.sites.begin());
+    }
+    ws.n_svd = 0; ws.sweeps = 0; ws.max_bond = 0;
+    mps.zip_right(window);
+    const int n = (int)mps.sites.size();
+    mps.sweep_left(n - 1, 0);
+    for (auto& nd : owned) free_node(nd, st());
+    // join prop_2: site[e][p] = P2[p][e]
+    {
+      std::vector<cplx> site((size_t)d2 * d2);
+      for (int e = 0; e < d2; ++e)
+        for (int p = 0; p < d2; ++p) site[(size_t)e * d2 + p] = P2[(size_t)p * d2 + e];
+      require(mps.sites.back().d[2] == d2, PTE_ERR_ARG, "internal: east leg is not d^2");
+      mps.sites.push_back(dupload(site, d2, d2, 1, 1, st()));
+    }
+    // read out: sum every physical leg but the newest
+    DevBuf vec, vec2;
+    std::vector<cplx> one(1, cplx(1.0, 0.0));
+    vec.ensure(1, st());
+    HIP_CHECK(hipMemcpyAsync(vec.p, one.data(), sizeof(double2), hipMemcpyHostToDevice,
+                             st()));
+    int len = 1;
+    for (size_t i = 0; i + 1 < mps.sites.size(); ++i) {
+      const DTensor& A = mps.sites[i];
+      const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
+      require(Dl == len && P == cfg.n_north, PTE_ERR_ARG, "internal: readout shape");
+      // tmp[p][r] = sum_l vec[l] A[l][p][r]; vec'[r] = sum_p tmp[p][r]
+      ws.G.ensure((size_t)P * Dr, st());
+      ws.gemm(1, P * Dr, Dl, vec.p, Dl, 1, A.p, (long)P * Dr, 1, ws.G.p, (long)P * Dr, 1);
+      vec2.ensure(Dr, st());
+      leg_vector_kernel<<<grid_for(Dr), 256, 0, st()>>>(ws.G.p, 1, P, Dr, sum_north_d.p,
+                                                        vec2.p);
+      std::swap(vec, vec2);
+      len = Dr;
+    }
+    const DTensor& L = mps.sites.back();
+    require(L.d[0] == len, PTE_ERR_ARG, "internal: readout shape (last)");
+    DevBuf outv;
+    outv.ensure(d2, st());
+    ws.gemm(1, d2, len, vec.p, len, 1, L.p, d2, 1, outv.p, d2, 1);
+    std::vector<cplx> h(d2);
+    HIP_CHECK(hipMemcpyAsync(h.data(), outv.p, d2 * sizeof(double2),
+                             hipMemcpyDeviceToHost, st()));
+    HIP_CHECK(hipStreamSynchronize(st()));
+    vec.release(st()); vec2.release(st()); outv.release(st());
+    for (int i = 0; i < d2; ++i) {
+      state_out[2 * i] = h[i].real();
+      state_out[2 * i + 1] = h[i].imag();
+    }
+  }
+
+  void destroy() {
+    mps.clear();
+    for (auto& nd : mpo) free_node(nd, st());
+    mpo.clear();
+    dfree(sum_north_d, st());
+    ws.destroy();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// compute_dynamics kernels (system_dynamics.py:631-700)
+// ---------------------------------------------------------------------------
+// w[a][i] = sum_j (sum_q v[a][q] P1[j][q]) Tin[j][i]  (Tin = identity if null)
+__global__ void __launch_bounds__(256)
+dyn_pre_kernel(const double2* v, int A, int d2, const double2* P1, const double2* tin,
+               double2* w) {
+  const long total = (long)A * d2;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = e % d2;
+    const long a = e / d2;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < d2; ++j) {
+      double2 pj = make_double2(0.0, 0.0);
+      for (int q = 0; q < d2; ++q) cfma(pj, P1[j * d2 + q], v[a * d2 + q]);
+      if (tin) cfma(acc, pj, tin[j * d2 + i]);
+      else if (j == i) acc = pj;
+    }
+    w[e] = acc;
+  }
+}
+
+// u[b][i] = sum_a w[a][i] T[a][i][b] * scale; v'[b][o] = sum_i u[b][i] Tout[i][o]
+// then v''[b][o2] = sum_o P2[o2][o] v'[b][o].  One thread per b; the site is
+// read in its [a][i][b] layout (coalesced over b).
+__global__ void __launch_bounds__(256)
+dyn_mpo_kernel(const double2* w, const double2* T, int A, int d2, int B, double scale,
+               const double2* tout, const double2* P2, double2* vout) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= B) return;
+  double2 u[16];
+  for (int i = 0; i < d2; ++i) u[i] = make_double2(0.0, 0.0);
+  for (int a = 0; a < A; ++a)
+    for (int i = 0; i < d2; ++i)
+      cfma(u[i], w[(long)a * d2 + i], T[((long)a * d2 + i) * B + b]);
+  double2 vp[16];
+  for (int o = 0; o < d2; ++o) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < d2; ++i) {
+      if (tout) cfma(acc, u[i], tout[i * d2 + o]);
+      else if (i == o) acc = u[i];
+    }
+    vp[o] = cscale(acc, scale);
+  }
+  for (int o2 = 0; o2 < d2; ++o2) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int o = 0; o < d2; ++o) cfma(acc, P2[o2 * d2 + o], vp[o]);
+    vout[(long)b * d2 + o2] = acc;
+  }
+}
+
+// rho[o] = sum_a cap[a] v[a][o]   (single workgroup)
+__global__ void __launch_bounds__(256)
+dyn_readout_kernel(const double2* v, const double2* cap, int A, int d2, double2* rho) {
+  __shared__ double2 red[256];
+  for (int o = 0; o < d2; ++o) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int a = threadIdx.x; a < A; a += 256) cfma(acc, cap[a], v[(long)a * d2 + o]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) rho[o] = red[0];
+    __syncthreads();
+  }
+}
+
+// Runs the dynamics over device-resident sites [a][i][b] (scaled by `scale`)
+// and caps.
+static void run_dynamics(hipStream_t st, int d2, int N, const std::vector<const double2*>& sites,
+                         const std::vector<int>& bonds, const std::vector<const double2*>& caps,
+                         const double* props, const double* tin_h, const double* tout_h,
+                         const double* rho0, double scale, double* states_out) {
+  require(d2 >= 1 && d2 <= 16, PTE_ERR_ARG, "Liouville dimension must be 1..16");
+  const long pstride = 2L * d2 * d2;
+  auto up = [&](const double* h, long count) {
+    double2* p = nullptr;
+    HIP_CHECK(hipMallocAsync((void**)&p, std::max<long>(count, 1) * sizeof(double2), st));
+    HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(double2), hipMemcpyHostToDevice, st));
+    return p;
+  };
+  for (long i = 0; i < 2L * N * d2 * d2 * 2; ++i)
+    require(std::isfinite(props[i]), PTE_ERR_NUMERIC, "non-finite propagator");
+  double2* dprops = up(props, 2L * N * d2 * d2);
+  double2* tin = tin_h ? up(tin_h, (long)d2 * d2) : nullptr;
+  double2* tout = tout_h ? up(tout_h, (long)d2 * d2) : nullptr;
+  int maxb = 1;
+  for (int b : bonds) maxb = std::max(maxb, b);
+  double2 *v = nullptr, *v2 = nullptr, *w = nullptr, *rho = nullptr;
+  HIP_CHECK(hipMallocAsync((void**)&v, (size_t)maxb * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&v2, (size_t)maxb * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&w, (size_t)maxb * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&rho, (size_t)(N + 1) * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMemcpyAsync(v, rho0, d2 * sizeof(double2), hipMemcpyHostToDevice, st));
+  (void)pstride;
+  for (int k = 0; k < N; ++k) {
+    const int A = bonds[k], B = bonds[k + 1];
+    dyn_readout_kernel<<<1, 256, 0, st>>>(v, caps[k], A, d2, rho + (long)k * d2);
+    const double2* P1 = dprops + (long)k * 2 * d2 * d2;
+    const double2* P2 = P1 + d2 * d2;
+    dyn_pre_kernel<<<grid_for((long)A * d2), 256, 0, st>>>(v, A, d2, P1, tin, w);
+    dyn_mpo_kernel<<<(B + 255) / 256, 256, 0, st>>>(w, sites[k], A, d2, B, scale, tout, P2,
+                                                     v2);
+    std::swap(v, v2);
+  }
+  dyn_readout_kernel<<<1, 256, 0, st>>>(v, caps[N], bonds[N], d2, rho + (long)N * d2);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(states_out, rho, (size_t)(N + 1) * d2 * sizeof(double2),
+                           hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  (void)hipFreeAsync(v, st); (void)hipFreeAsync(v2, st); (void)hipFreeAsync(w, st); (void)hipFreeAsync(rho, st);
+  (void)hipFreeAsync(dprops, st);
+  if (tin) (void)hipFreeAsync(tin, st);
+  if (tout) (void)hipFreeAsync(tout, st);
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return PTE_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return PTE_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return PTE_ERR_DEVICE;
+  }
+}
+
+}  // namespace pte
+
+using namespace pte;
+
+struct pte_pt_engine { PtEngine impl; };
+struct pte_tempo_engine { TempoEngine impl; };
+
+static void store_infl(HostInfl& h, const double* data, int rows, int cols) {
+  require(data != nullptr && rows > 0 && cols >= 0, PTE_ERR_ARG, "bad influence shape");
+  const long count = (cols == 0) ? rows : (long)rows * cols;
+  h.v = to_cplx(data, count);
+  h.rows = rows;
+  h.cols = cols;
+  h.set = true;
+}
+
+extern "C" {
+
+const char* pte_last_error(void) { return g_last_error.c_str(); }
+const char* pte_version(void) { return "ptengine 0.1 gfx950"; }
+
+int pte_pt_create(const pte_pt_config* cfg, pte_pt_engine** out) {
+  return guarded([&] {
+    require(cfg && out, PTE_ERR_ARG, "null argument");
+    require(cfg->dimension >= 1 && cfg->dimension <= 4, PTE_ERR_ARG,
+            "dimension must be 1..4");
+    require(cfg->num_steps >= 2, PTE_ERR_ARG, "num_steps must be >= 2");
+    require(cfg->dkmax >= 1, PTE_ERR_ARG, "dkmax must be >= 1");
+    require(cfg->epsrel > 0.0 && std::isfinite(cfg->epsrel), PTE_ERR_ARG,
+            "epsrel must be positive");
+    require(cfg->precision == PTE_PRECISION_C128, PTE_ERR_ARG,
+            "only the complex128 precision is available");
+    require(cfg->chi_cap >= 1, PTE_ERR_ARG, "chi_cap must be positive");
+    require(cfg->n_north >= 1, PTE_ERR_ARG, "n_north must be positive");
+    auto* e = new pte_pt_engine();
+    e->impl.cfg = *cfg;
+    try {
+      e->impl.init_engine();
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+int pte_pt_set_influence(pte_pt_engine* e, int dk, const double* data, int rows, int cols) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    require(dk >= 0 && dk < e->impl.num_infl, PTE_ERR_ARG, "dk out of range");
+    store_infl(e->impl.infl[dk], data, rows, cols);
+  });
+}
+
+int pte_pt_set_degeneracy_maps(pte_pt_engine* e, const int* nmap, const int* wmap, int length) {
+  return guarded([&] {
+    require(e && nmap && wmap, PTE_ERR_ARG, "null argument");
+    const int d2 = e->impl.cfg.dimension * e->impl.cfg.dimension;
+    require(length == d2, PTE_ERR_ARG, "degeneracy maps must have length d^2");
+    e->impl.north_map.assign(nmap, nmap + length);
+    e->impl.west_map.assign(wmap, wmap + length);
+    for (int i = 0; i < length; ++i)
+      require(nmap[i] >= 0 && nmap[i] < d2 && wmap[i] >= 0 && wmap[i] < d2, PTE_ERR_ARG,
+              "degeneracy map entry out of range");
+  });
+}
+
+int pte_pt_initialize(pte_pt_engine* e) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    require(e->impl.step == 0, PTE_ERR_STATE, "already initialised");
+    e->impl.initialize();
+  });
+}
+
+int pte_pt_step(pte_pt_engine* e, const double* last, int rows, int cols, int* more) {
+  return guarded([&] {
+    require(e && more, PTE_ERR_ARG, "null argument");
+    *more = e->impl.compute_step(last, rows, cols) ? 1 : 0;
+  });
+}
+
+int pte_pt_get_step(pte_pt_engine* e, int* step) {
+  return guarded([&] {
+    require(e && step, PTE_ERR_ARG, "null argument");
+    *step = e->impl.step;
+  });
+}
+
+int pte_pt_num_sites(pte_pt_engine* e, int* n) {
+  return guarded([&] {
+    require(e && n, PTE_ERR_ARG, "null argument");
+    *n = (int)e->impl.mps.sites.size();
+  });
+}
+
+int pte_pt_bond_dims(pte_pt_engine* e, int* out, int n) {
+  return guarded([&] {
+    require(e && out, PTE_ERR_ARG, "null argument");
+    const auto& s = e->impl.mps.sites;
+    require(n == (int)s.size() - 1, PTE_ERR_ARG, "n must be n_sites - 1");
+    for (int i = 0; i < n; ++i) out[i] = s[i].d[2];
+  });
+}
+
+int pte_pt_get_mpo_tensor(pte_pt_engine* e, int step, double* out, long cap, int* shape3) {
+  return guarded([&] {
+    require(e && out && shape3, PTE_ERR_ARG, "null argument");
+    e->impl.get_mpo_tensor(step, out, cap, shape3);
+  });
+}
+
+int pte_pt_compute_caps(pte_pt_engine* e) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    e->impl.compute_caps();
+  });
+}
+
+int pte_pt_get_cap(pte_pt_engine* e, int k, double* out, int cap, int* length) {
+  return guarded([&] {
+    require(e && out && length, PTE_ERR_ARG, "null argument");
+    auto& caps = e->impl.caps;
+    require(!caps.empty(), PTE_ERR_STATE, "caps not computed");
+    require(k >= 0 && k < (int)caps.size(), PTE_ERR_ARG, "cap index out of range");
+    const int n = caps[k].d[0];
+    *length = n;
+    require(cap >= n, PTE_ERR_ARG, "output buffer too small");
+    HIP_CHECK(hipMemcpyAsync(out, caps[k].p, n * sizeof(double2), hipMemcpyDeviceToHost,
+                             e->impl.st()));
+    HIP_CHECK(hipStreamSynchronize(e->impl.st()));
+  });
+}
+
+int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps, int* max_bond,
+                           double* ms) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    if (n_svd) *n_svd = e->impl.ws.n_svd;
+    if (sweeps) *sweeps = e->impl.ws.sweeps;
+    if (max_bond) *max_bond = e->impl.ws.max_bond;
+    if (ms) *ms = e->impl.last_ms;
+  });
+}
+
+void pte_pt_destroy(pte_pt_engine* e) {
+  if (!e) return;
+  guarded([&] { e->impl.destroy(); });
+  delete e;
+}
+
+int pte_tempo_create(const pte_tempo_config* cfg, pte_tempo_engine** out) {
+  return guarded([&] {
+    require(cfg && out, PTE_ERR_ARG, "null argument");
+    require(cfg->dimension >= 1 && cfg->dimension <= 4, PTE_ERR_ARG,
+            "dimension must be 1..4");
+    require(cfg->dkmax >= -1, PTE_ERR_ARG, "dkmax must be >= 0 or -1");
+    require(cfg->epsrel > 0.0 && std::isfinite(cfg->epsrel), PTE_ERR_ARG,
+            "epsrel must be positive");
+    require(cfg->precision == PTE_PRECISION_C128, PTE_ERR_ARG,
+            "only the complex128 precision is available");
+    require(cfg->chi_cap >= 1 && cfg->n_north >= 1 && cfg->n_west >= 1, PTE_ERR_ARG,
+            "bad capacity or vector lengths");
+    auto* e = new pte_tempo_engine();
+    e->impl.cfg = *cfg;
+    try {
+      e->impl.init_engine();
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+int pte_tempo_set_influence(pte_tempo_engine* e, int dk, const double* data, int rows,
+                            int cols) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    require(dk >= 0, PTE_ERR_ARG, "dk must be non-negative");
+    store_infl(e->impl.slot(dk), data, rows, cols);
+  });
+}
+
+int pte_tempo_set_degeneracy_maps(pte_tempo_engine* e, const int* nmap, const int* wmap,
+                                  int length) {
+  return guarded([&] {
+    require(e && nmap && wmap, PTE_ERR_ARG, "null argument");
+    const int d2 = e->impl.cfg.dimension * e->impl.cfg.dimension;
+    require(length == d2, PTE_ERR_ARG, "degeneracy maps must have length d^2");
+    e->impl.north_map.assign(nmap, nmap + length);
+    e->impl.west_map.assign(wmap, wmap + length);
+  });
+}
+
+int pte_tempo_initialize(pte_tempo_engine* e, const double* rho0, const double* unitary) {
+  return guarded([&] {
+    require(e && rho0 && unitary, PTE_ERR_ARG, "null argument");
+    require(e->impl.step < 0, PTE_ERR_STATE, "already initialised");
+    e->impl.initialize(rho0, unitary);
+  });
+}
+
+int pte_tempo_step(pte_tempo_engine* e, const double* p1, const double* p2,
+                   const double* extra, int rows, int cols, double* state_out) {
+  return guarded([&] {
+    require(e && p1 && p2 && state_out, PTE_ERR_ARG, "null argument");
+    e->impl.compute_step(p1, p2, extra, rows, cols, state_out);
+  });
+}
+
+int pte_tempo_get_step(pte_tempo_engine* e, int* step) {
+  return guarded([&] {
+    require(e && step, PTE_ERR_ARG, "null argument");
+    *step = e->impl.step;
+  });
+}
+
+int pte_tempo_bond_dims(pte_tempo_engine* e, int* out, int n, int* n_sites) {
+  return guarded([&] {
+    require(e && n_sites, PTE_ERR_ARG, "null argument");
+    const auto& s = e->impl.mps.sites;
+    *n_sites = (int)s.size();
+    if (out) {
+      require(n >= (int)s.size() - 1, PTE_ERR_ARG, "output too small");
+      for (size_t i = 0; i + 1 < s.size(); ++i) out[i] = s[i].d[2];
+    }
+  });
+}
+
+void pte_tempo_destroy(pte_tempo_engine* e) {
+  if (!e) return;
+  guarded([&] { e->impl.destroy(); });
+  delete e;
+}
+
+int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap, double* U,
+                  double* S, double* SVh, int* k_out, int* sweeps_out) {
+  return guarded([&] {
+    require(m >= 1 && n >= 1 && M && U && S && SVh && k_out, PTE_ERR_ARG,
+            "bad arguments");
+    require(epsrel > 0.0 && chi_cap >= 1, PTE_ERR_ARG, "bad epsrel or capacity");
+    Workspace ws;
+    ws.init(epsrel, chi_cap);
+    try {
+      const long cnt = (long)m * n;
+      std::vector<cplx> h = to_cplx(M, cnt);
+      DTensor dm = dupload(h, m, n, 1, 1, ws.stream);
+      ws.X.ensure(cnt, ws.stream);
+      // X = M^H (n x m, column-major) is the elementwise conjugate of M
+      conj_copy_kernel<<<grid_for(cnt), 256, 0, ws.stream>>>(dm.p, cnt, ws.X.p);
+      const int k = ws.svd(n, m);
+      DevBuf u, svh;
+      u.ensure((size_t)m * k, ws.stream);
+      svh.ensure((size_t)k * n, ws.stream);
+      ws.emit(ws.V.p, m, m, 1, 1, 0, 0, 1, 0, 0, 0, u.p, k);
+      ws.emit(ws.X.p, n, n, 1, n, 0, 1, 0, 0, 0, 1, svh.p, k);
+      HIP_CHECK(hipMemcpyAsync(U, u.p, (size_t)m * k * sizeof(double2),
+                               hipMemcpyDeviceToHost, ws.stream));
+      HIP_CHECK(hipMemcpyAsync(SVh, svh.p, (size_t)k * n * sizeof(double2),
+                               hipMemcpyDeviceToHost, ws.stream));
+      HIP_CHECK(hipMemcpyAsync(S, ws.sorted, (size_t)k * sizeof(double),
+                               hipMemcpyDeviceToHost, ws.stream));
+      HIP_CHECK(hipStreamSynchronize(ws.stream));
+      *k_out = k;
+      if (sweeps_out) *sweeps_out = ws.info_h->sweeps;
+      u.release(ws.stream);
+      svh.release(ws.stream);
+      dfree(dm, ws.stream);
+    } catch (...) {
+      ws.destroy();
+      throw;
+    }
+    ws.destroy();
+  });
+}
+
+int pte_dynamics_run(int d2, int num_steps, const int* bonds, const double* tensors,
+                     const double* caps, const double* props, const double* tin,
+                     const double* tout, const double* rho0, double* states_out) {
+  return guarded([&] {
+    require(bonds && tensors && caps && props && rho0 && states_out, PTE_ERR_ARG,
+            "null argument");
+    require(num_steps >= 1, PTE_ERR_ARG, "num_steps must be >= 1");
+    require(bonds[0] == 1, PTE_ERR_ARG, "the first bond must be 1");
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::vector<int> b(bonds, bonds + num_steps + 1);
+    std::vector<double2*> dsites(num_steps), dcaps(num_steps + 1);
+    long off = 0, coff = 0;
+    for (int k = 0; k < num_steps; ++k) {
+      require(b[k] >= 1 && b[k + 1] >= 1, PTE_ERR_ARG, "bond dims must be positive");
+      // tensors come as [a][b][i]; the kernel reads [a][i][b]
+      const long cnt = (long)b[k] * b[k + 1] * d2;
+      std::vector<cplx> h(cnt);
+      for (int a = 0; a < b[k]; ++a)
+        for (int bb = 0; bb < b[k + 1]; ++bb)
+          for (int i = 0; i < d2; ++i) {
+            const long src = off + ((long)a * b[k + 1] + bb) * d2 + i;
+            h[((long)a * d2 + i) * b[k + 1] + bb] = cplx(tensors[2 * src], tensors[2 * src + 1]);
+          }
+      HIP_CHECK(hipMallocAsync((void**)&dsites[k], cnt * sizeof(double2), st));
+      HIP_CHECK(hipMemcpyAsync(dsites[k], h.data(), cnt * sizeof(double2),
+                               hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      off += cnt;
+    }
+    require(b[num_steps] >= 1, PTE_ERR_ARG, "bond dims must be positive");
+    for (int k = 0; k <= num_steps; ++k) {
+      HIP_CHECK(hipMallocAsync((void**)&dcaps[k], b[k] * sizeof(double2), st));
+      HIP_CHECK(hipMemcpyAsync(dcaps[k], caps + 2 * coff, b[k] * sizeof(double2),
+                               hipMemcpyHostToDevice, st));
+      coff += b[k];
+    }
+    std::vector<const double2*> cs(dsites.begin(), dsites.end());
+    std::vector<const double2*> cc(dcaps.begin(), dcaps.end());
+    run_dynamics(st, d2, num_steps, cs, b, cc, props, tin, tout, rho0, 1.0, states_out);
+    for (auto p : dsites) (void)hipFreeAsync(p, st);
+    for (auto p : dcaps) (void)hipFreeAsync(p, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    (void)hipStreamDestroy(st);
+  });
+}
+
+int pte_dynamics_run_engine(pte_pt_engine* e, const double* props, const double* tin,
+                            const double* tout, const double* rho0, double* states_out) {
+  return guarded([&] {
+    require(e && props && rho0 && states_out, PTE_ERR_ARG, "null argument");
+    PtEngine& pe = e->impl;
+    require(pe.step >= pe.cfg.num_steps, PTE_ERR_STATE, "process tensor not complete");
+    require(!pe.caps.empty(), PTE_ERR_STATE, "caps not computed");
+    const int N = pe.cfg.num_steps, d2 = pe.cfg.dimension * pe.cfg.dimension;
+    std::vector<int> b(N + 1);
+    std::vector<const double2*> sites(N), caps(N + 1);
+    for (int k = 0; k < N; ++k) {
+      b[k] = pe.mps.sites[k].d[0];
+      sites[k] = pe.mps.sites[k].p;
+      require(pe.mps.sites[k].d[1] == d2, PTE_ERR_ARG, "internal: site leg is not d^2");
+    }
+    b[N] = pe.mps.sites[N - 1].d[2];
+    for (int k = 0; k <= N; ++k) caps[k] = pe.caps[k].p;
+    run_dynamics(pe.st(), d2, N, sites, b, caps, props, tin, tout, rho0,
+                 (double)pe.cfg.dimension, states_out);
+  });
+}
+
+}  // extern "C"

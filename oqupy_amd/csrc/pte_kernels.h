@@ -1,0 +1,80 @@
+// Argument blocks and kernel declarations of the process-tensor engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pte {
+
+enum NodeKind { NODE_DENSE = 0, NODE_DELTA = 1 };
+enum MpoOut { MPO_OUT_SVD_H = 0, MPO_OUT_SITE = 1 };
+
+struct GemmArgs {
+  int M, N, K;
+  const double2* A; long sai, sak;
+  const double2* B; long sbk, sbj;
+  double2* C; long sci, scj;
+  int conjA, conjB;
+  double2 alpha;
+  int beta_zero;
+};
+
+struct MpoApplyArgs {
+  const double2* G;   // GEMM result
+  const double2* W;   // dense node [w][p][s][v]
+  const double2* I;   // delta node weights [w][p]
+  int kind, mode;
+  int Dl, P, Dx, V, Wl, S;
+  double2* out;
+  long ldx;
+};
+
+// Summary of one truncated SVD, written by the device.
+struct SvdInfo {
+  int rank;        // kept singular values
+  int overflow;    // rank above the bond capacity
+  int sweeps;      // Jacobi sweeps (or -1 if not converged)
+  int pad;
+  double s0;       // largest singular value
+  double discarded;  // 2-norm of the discarded singular values
+};
+
+struct JacobiArgs {
+  double2* X; int m, n; long ldx;
+  double2* V; long ldv;
+  int nb;
+  double tol, abs_tol_scale;
+  int max_sweeps;
+  unsigned* bar;
+  int* flags;
+  double* fro2;
+  int* info;
+};
+
+struct EmitArgs {
+  const double2* src; long lds;
+  const int* perm;
+  const SvdInfo* info;
+  int A, B;
+  long dj0, djk, da0, dak, db0, dbk;
+  int conj;
+  double2* dst;
+};
+
+__global__ void zgemm_kernel(GemmArgs a);
+__global__ void mpo_apply_kernel(MpoApplyArgs a);
+__global__ void mpo_apply_right_kernel(MpoApplyArgs a);
+__global__ void jacobi_kernel(JacobiArgs a);
+__global__ void column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma);
+__global__ void rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted);
+__global__ void truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
+                                int max_rank);
+__global__ void emit_kernel(EmitArgs a);
+__global__ void conj_copy_kernel(const double2* src, long count, double2* dst);
+__global__ void conj_transpose_kernel(const double2* src, int R, int C, double2* dst);
+__global__ void leg_vector_kernel(const double2* T, int A, int P, int B, const double2* vec,
+                                  double2* out);
+__global__ void identity_carry_kernel(double2* dst, int R, int V);
+
+constexpr int kJacobiBlock = 16;
+
+}  // namespace pte

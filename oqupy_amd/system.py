@@ -1,0 +1,207 @@
+"""Systems and their half-step propagators.
+
+Restates ``System`` and ``TimeDependentSystem`` of ``oqupy/system.py:48-272``
+and the Lindblad Liouvillian of ``oqupy/system.py:1118-1126``. Propagators are
+``expm(L dt/2)`` super-operators in the row-major vectorisation convention;
+they stay on the host (d^2 x d^2, FP64) and are handed to the engine as
+arrays.
+"""
+
+from copy import copy
+from functools import lru_cache
+from typing import Callable, List, Optional, Text
+
+import numpy as np
+from numpy import ndarray
+from scipy import integrate
+from scipy.linalg import expm
+
+from oqupy_amd import operators as opr
+from oqupy_amd.base_api import BaseAPIClass
+from oqupy_amd.config import NpDtype
+
+
+def _liouvillian(hamiltonian, gammas, lindblad_operators) -> ndarray:
+    """``L rho = -i[H, rho] + sum_n g_n (A rho A^+ - 1/2 {A^+ A, rho})``
+    (oqupy/system.py:1118-1126)."""
+    liou = -1j * opr.commutator(hamiltonian)
+    for gamma, op in zip(gammas, lindblad_operators):
+        op_dag = op.conjugate().T
+        liou = liou + gamma * (opr.left_right_super(op, op_dag)
+                               - 0.5 * opr.acommutator(op_dag @ op))
+    return liou
+
+
+def _check_hamiltonian(hamiltonian) -> ndarray:
+    """oqupy/system.py:969-982."""
+    try:
+        ham = np.array(hamiltonian, dtype=NpDtype)
+        ham.setflags(write=False)
+    except Exception as err:
+        raise AssertionError("Coupling operator must be numpy array") from err
+    assert ham.ndim == 2, "Coupling operator is not a matrix."
+    assert ham.shape[0] == ham.shape[1], \
+        "Coupling operator is not a square matrix."
+    return ham
+
+
+def _check_dissipator_lists(gammas, lindblad_operators):
+    """oqupy/system.py:1006-1019."""
+    gammas = [] if gammas is None else gammas
+    lindblad_operators = [] if lindblad_operators is None \
+        else lindblad_operators
+    assert isinstance(gammas, list), "Argument `gammas` must be a list)]."
+    assert isinstance(lindblad_operators, list), \
+        "Argument `lindblad_operators` must be a list."
+    assert len(gammas) == len(lindblad_operators), \
+        "Lists `gammas` and `lindblad_operators` must have the same length."
+    return gammas, lindblad_operators
+
+
+class BaseSystem(BaseAPIClass):
+    """Base class of systems (oqupy/system.py:31-46)."""
+
+    def __init__(self, dimension: int, name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        self._dimension = dimension
+        super().__init__(name, description)
+
+    @property
+    def dimension(self) -> int:
+        """Hilbert space dimension."""
+        return self._dimension
+
+
+class System(BaseSystem):
+    """A time-independent Hamiltonian with optional Lindblad terms
+    (oqupy/system.py:48-154)."""
+
+    def __init__(self, hamiltonian: ndarray,
+                 gammas: Optional[List[float]] = None,
+                 lindblad_operators: Optional[List[ndarray]] = None,
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        self._hamiltonian = _check_hamiltonian(hamiltonian)
+        gammas, lindblad_operators = _check_dissipator_lists(
+            gammas, lindblad_operators)
+        try:
+            self._gammas = [float(g) for g in gammas]
+        except Exception as err:
+            raise AssertionError("All elements of `gammas` must be floats.") \
+                from err
+        try:
+            self._lindblad_operators = [np.array(op, dtype=NpDtype)
+                                        for op in lindblad_operators]
+        except Exception as err:
+            raise AssertionError("All elements of `lindblad_operators` must "
+                                 "be numpy arrays.") from err
+        super().__init__(self._hamiltonian.shape[0], name, description)
+
+    @lru_cache(4)
+    def liouvillian(self) -> ndarray:
+        """The Liouvillian super-operator."""
+        return _liouvillian(self._hamiltonian, self._gammas,
+                            self._lindblad_operators)
+
+    def get_propagators(self, dt, start_time, subdiv_limit, epsrel):
+        """Half-step propagators, the same for every step
+        (oqupy/system.py:121-129)."""
+        first = expm(self.liouvillian() * dt / 2.0)
+        second = expm(self.liouvillian() * dt / 2.0)
+        return lambda step: (first, second)
+
+    @property
+    def hamiltonian(self) -> ndarray:
+        """The Hamiltonian."""
+        return copy(self._hamiltonian)
+
+    @property
+    def gammas(self) -> List[float]:
+        """Lindblad rates."""
+        return copy(self._gammas)
+
+    @property
+    def lindblad_operators(self) -> List[ndarray]:
+        """Lindblad operators."""
+        return copy(self._lindblad_operators)
+
+
+class TimeDependentSystem(BaseSystem):
+    """A time-dependent Hamiltonian with optional time-dependent Lindblad
+    terms (oqupy/system.py:156-286)."""
+
+    def __init__(self, hamiltonian: Callable[[float], ndarray],
+                 gammas: Optional[List[Callable[[float], float]]] = None,
+                 lindblad_operators: Optional[
+                     List[Callable[[float], ndarray]]] = None,
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        try:
+            dim = _check_hamiltonian(hamiltonian(1.0)).shape[0]
+        except Exception as err:
+            raise AssertionError("Time dependent Hamiltonian must be "
+                                 "vectorizable callable.") from err
+        self._hamiltonian = hamiltonian
+        gammas, lindblad_operators = _check_dissipator_lists(
+            gammas, lindblad_operators)
+        try:
+            for gamma in gammas:
+                float(gamma(1.0))
+        except Exception as err:
+            raise AssertionError("All elements of `gammas` must be "
+                                 "vectorizable callables returning floats.") \
+                from err
+        try:
+            for op in lindblad_operators:
+                np.array(op(1.0))
+        except Exception as err:
+            raise AssertionError("All elements of `lindblad_operators` must "
+                                 "be vectorizable callables returning numpy "
+                                 "arrays.") from err
+        self._gammas = list(gammas)
+        self._lindblad_operators = list(lindblad_operators)
+        super().__init__(dim, name, description)
+
+    def liouvillian(self, t: float) -> ndarray:
+        """The Liouvillian at time ``t``."""
+        ham = np.array(self._hamiltonian(t), dtype=NpDtype)
+        gammas = [float(g(t)) for g in self._gammas]
+        ops = [np.array(op(t), dtype=NpDtype) for op in
+               self._lindblad_operators]
+        return _liouvillian(ham, gammas, ops)
+
+    def get_propagators(self, dt, start_time, subdiv_limit, epsrel):
+        """Half-step propagators of step ``step``: the Liouvillian sampled at
+        the quarter points (``subdiv_limit is None``) or integrated over each
+        half step with ``quad_vec`` (oqupy/system.py:240-272)."""
+        if subdiv_limit is None:
+            def propagators(step: int):
+                t = start_time + step * dt
+                return (expm(self.liouvillian(t + dt / 4.0) * dt / 2.0),
+                        expm(self.liouvillian(t + dt * 3.0 / 4.0) * dt / 2.0))
+        else:
+            def propagators(step: int):
+                t = start_time + step * dt
+                first = integrate.quad_vec(self.liouvillian, a=t,
+                                           b=t + dt / 2.0, epsrel=epsrel,
+                                           limit=subdiv_limit)[0]
+                second = integrate.quad_vec(self.liouvillian, a=t + dt / 2.0,
+                                            b=t + dt, epsrel=epsrel,
+                                            limit=subdiv_limit)[0]
+                return expm(first), expm(second)
+        return propagators
+
+    @property
+    def hamiltonian(self) -> Callable[[float], ndarray]:
+        """The Hamiltonian callable."""
+        return copy(self._hamiltonian)
+
+    @property
+    def gammas(self):
+        """Lindblad rate callables."""
+        return copy(self._gammas)
+
+    @property
+    def lindblad_operators(self):
+        """Lindblad operator callables."""
+        return copy(self._lindblad_operators)

@@ -1,0 +1,172 @@
+"""``compute_dynamics``: system dynamics through a process tensor.
+
+Restates ``compute_dynamics`` and its input parsing from
+``oqupy/system_dynamics.py:41-182, 478-559`` for one (or no) process tensor
+and no controls. The per-step contraction -- first half-step propagator,
+PT-MPO tensor (with the basis transforms of a non-diagonal coupling), second
+half-step, and the cap readout of every state -- runs in the MI355X engine
+(``pte_dynamics_run`` / ``pte_dynamics_run_engine``).
+"""
+
+from typing import List, Optional, Text, Union
+
+import numpy as np
+from numpy import ndarray
+
+from oqupy_amd.backends import engine as eng
+from oqupy_amd.config import INTEGRATE_EPSREL, SUBDIV_LIMIT
+from oqupy_amd.dynamics import Dynamics
+from oqupy_amd.process_tensor import BaseProcessTensor, TrivialProcessTensor
+from oqupy_amd.system import System, TimeDependentSystem
+from oqupy_amd.util import check_convert, check_isinstance, check_true
+
+
+def _compute_dynamics_input_parse(system, initial_state, dt, num_steps,
+                                  start_time, process_tensor, control,
+                                  record_all):
+    """oqupy/system_dynamics.py:478-559."""
+    check_isinstance(system, (System, TimeDependentSystem), "system")
+    hs_dim = system.dimension
+    if process_tensor is None:
+        process_tensor = []
+    check_isinstance(process_tensor, (BaseProcessTensor, list),
+                     "process_tensor")
+    process_tensors = [process_tensor] \
+        if isinstance(process_tensor, BaseProcessTensor) else process_tensor
+    if control is not None:
+        raise NotImplementedError("Controls are not supported by this "
+                                  "package's compute_dynamics.")
+    start_time = check_convert(start_time, float, "start_time")
+    if initial_state is None:
+        raise ValueError("An initial state must be specified.")
+    check_isinstance(initial_state, ndarray, "initial_state")
+    check_true(initial_state.shape == (hs_dim, hs_dim),
+               "Initial sate must be a square matrix of "
+               f"dimension {hs_dim}x{hs_dim}.")
+    max_steps = []
+    for pt in process_tensors:
+        check_isinstance(pt, BaseProcessTensor, "pt",
+                         "One of the elements in `process_tensor` is not of "
+                         f"type `{BaseProcessTensor.__name__}`.")
+        if pt.get_initial_tensor() is not None:
+            raise NotImplementedError()
+        check_true(hs_dim == pt.hilbert_space_dimension,
+                   "All process tensor must have the same Hilbert space "
+                   "dimension as the system.")
+        if pt.dt is not None:
+            if dt is None:
+                dt = pt.dt
+            else:
+                check_true(pt.dt == dt, "All process tensors must have the "
+                           "same timestep length.")
+        max_steps.append(pt.max_step)
+    max_step = np.min(max_steps + [np.inf])
+    if dt is None:
+        raise ValueError(
+            "No timestep length has been specified. Please set `dt`.")
+    if num_steps is not None:
+        num_steps = check_convert(num_steps, int, "num_steps")
+        check_true(num_steps <= max_step, "Variable `num_steps` is larger "
+                   "than the shortest process tensor!")
+    else:
+        check_true(max_step < np.inf, "Variable `num_steps` must be "
+                   "specified because all process tensors involved are "
+                   "infinite.")
+        num_steps = int(max_step)
+    return (system, initial_state, dt, num_steps, start_time, process_tensors,
+            record_all, hs_dim)
+
+
+def compute_dynamics(
+        system: Union[System, TimeDependentSystem],
+        initial_state: Optional[ndarray] = None,
+        dt: Optional[float] = None,
+        num_steps: Optional[int] = None,
+        start_time: Optional[float] = 0.0,
+        process_tensor: Optional[Union[List[BaseProcessTensor],
+                                       BaseProcessTensor]] = None,
+        control=None,
+        record_all: Optional[bool] = True,
+        subdiv_limit: Optional[int] = SUBDIV_LIMIT,
+        liouvillian_epsrel: Optional[float] = INTEGRATE_EPSREL,
+        progress_type: Optional[Text] = None) -> Dynamics:
+    """Dynamics of ``system`` coupled through ``process_tensor``
+    (oqupy/system_dynamics.py:41-182). States at ``start_time + k dt`` for
+    ``k = 0 .. num_steps`` (only the last one if ``record_all`` is False)."""
+    (system, initial_state, dt, num_steps, start_time, process_tensors,
+     record_all, hs_dim) = _compute_dynamics_input_parse(
+         system, initial_state, dt, num_steps, start_time, process_tensor,
+         control, record_all)
+    process_tensors = [pt for pt in process_tensors
+                       if not isinstance(pt, TrivialProcessTensor)]
+    if len(process_tensors) > 1:
+        raise NotImplementedError("More than one process tensor is not "
+                                  "supported by this package.")
+    d2 = hs_dim ** 2
+    propagators = system.get_propagators(dt, start_time, subdiv_limit,
+                                         liouvillian_epsrel)
+    props = np.empty((num_steps, 2, d2, d2), dtype=np.complex128)
+    for step in range(num_steps):
+        first, second = propagators(step)
+        props[step, 0] = first
+        props[step, 1] = second
+    rho0 = eng.as_c128(initial_state.reshape(d2))
+    states = np.empty((num_steps + 1, d2), dtype=np.complex128)
+    eng.require_gpu()
+    lib = eng.load_library()
+
+    if process_tensors:
+        pt = process_tensors[0]
+        tin = None if pt.transform_in is None else eng.as_c128(pt.transform_in)
+        tout = None if pt.transform_out is None \
+            else eng.as_c128(pt.transform_out)
+        backend = getattr(pt, "engine", None)
+        if backend is not None and num_steps == len(pt):
+            eng.check(lib.pte_dynamics_run_engine(
+                backend.handle, eng.dptr(props), eng.dptr(tin),
+                eng.dptr(tout), eng.dptr(rho0), eng.dptr(states)))
+        else:
+            tensors, bonds, caps = [], [], []
+            for step in range(num_steps):
+                ten = pt.get_raw_mpo_tensor(step)
+                if ten.ndim != 3:
+                    raise NotImplementedError(
+                        "Only rank-3 PT-MPO tensors are supported.")
+                tensors.append(eng.as_c128(ten).reshape(-1))
+                bonds.append(ten.shape[0])
+            bonds.append(pt.get_raw_mpo_tensor(num_steps - 1).shape[1]
+                         if num_steps == len(pt) else
+                         pt.get_raw_mpo_tensor(num_steps).shape[0])
+            for step in range(num_steps + 1):
+                cap = pt.get_cap_tensor(step)
+                if cap is None:
+                    raise ValueError(f"Process tensor 0 has no cap tensor for "
+                                     f"step {step}.")
+                caps.append(eng.as_c128(cap).reshape(-1))
+            _run_host(lib, d2, num_steps, bonds, tensors, caps, props, tin,
+                      tout, rho0, states)
+    else:
+        ones = [np.ones(d2, dtype=np.complex128) for _ in range(num_steps)]
+        caps = [np.ones(1, dtype=np.complex128)] * (num_steps + 1)
+        _run_host(lib, d2, num_steps, [1] * (num_steps + 1), ones, caps,
+                  props, None, None, rho0, states)
+
+    state_list = [s.reshape(hs_dim, hs_dim) for s in states]
+    if record_all:
+        times = start_time + np.arange(len(state_list)) * dt
+        return Dynamics(times=list(times), states=state_list)
+    # the reference records one state and dates it start_time + 1 * dt
+    # (oqupy/system_dynamics.py:173-180)
+    return Dynamics(times=[start_time + 1 * dt], states=[state_list[-1]])
+
+
+def _run_host(lib, d2, num_steps, bonds, tensors, caps, props, tin, tout,
+              rho0, states) -> None:
+    bonds_arr = np.ascontiguousarray(bonds, dtype=np.int32)
+    ten = np.ascontiguousarray(np.concatenate(tensors))
+    cap = np.ascontiguousarray(np.concatenate(caps))
+    eng.check(lib.pte_dynamics_run(d2, num_steps, eng.iptr(bonds_arr),
+                                   eng.dptr(ten), eng.dptr(cap),
+                                   eng.dptr(props), eng.dptr(tin),
+                                   eng.dptr(tout), eng.dptr(rho0),
+                                   eng.dptr(states)))

@@ -1,0 +1,193 @@
+"""Parity of the MI355X engine with the FP64 oracle (needs a GPU).
+
+Every test calls the engine through its C ABI (via the package API or the
+ctypes binding) and compares with the oracle fed the same influence arrays
+and propagators. Tolerances: singular values 1e-12 relative to s_0; states
+and expectation values 1e-10 absolute (complex128 plain form); bond
+dimensions identical.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oqupy_amd as oqupy
+from oqupy_amd import operators as op
+from oqupy_amd.backends import engine
+
+from tests import models
+
+pytestmark = pytest.mark.gpu
+
+STATE_TOL = 1e-10
+
+
+def _announce(name):
+    print(f"[gpu-case] {name}", flush=True)
+    sys.stdout.flush()
+
+
+def _graded(m, n, seed, decades=14):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
+    u, _, vh = np.linalg.svd(a, full_matrices=False)
+    s = 10.0 ** (-np.linspace(0, decades, min(m, n)))
+    return (u * s) @ vh
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (4, 4), (5, 3), (3, 5), (40, 30),
+                                 (30, 40), (64, 64), (100, 90), (260, 210)])
+def test_split_svd_matches_lapack(m, n):
+    _announce(f"split_svd {m}x{n}")
+    from oracle.chain import truncated_svd
+    mat = _graded(m, n, seed=m * 1000 + n)
+    eps = 1e-7
+    u, s, svh, sweeps = engine.split_svd(mat, eps)
+    u_ref, s_ref, vh_ref = truncated_svd(mat, eps)
+    assert len(s) == len(s_ref)
+    assert np.abs(s - s_ref).max() <= 1e-12 * s_ref[0]
+    approx_ref = (u_ref * s_ref) @ vh_ref
+    assert np.abs(u @ svh - approx_ref).max() <= 1e-12 * s_ref[0]
+    assert np.abs(u.conj().T @ u - np.eye(len(s))).max() < 1e-12
+    assert sweeps >= 1
+
+
+def test_split_svd_rank_deficient_and_clustered():
+    _announce("split_svd rank-deficient")
+    from oracle.chain import truncated_svd
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(80, 5)) + 1j * rng.normal(size=(80, 5))
+    b = rng.normal(size=(5, 70)) + 1j * rng.normal(size=(5, 70))
+    mat = a @ b
+    mat[:, 10] = 0.0
+    u, s, svh, _ = engine.split_svd(mat, 1e-9)
+    _, s_ref, _ = truncated_svd(mat, 1e-9)
+    assert len(s) == len(s_ref) == 5
+    assert np.abs(s - s_ref).max() <= 1e-12 * s_ref[0]
+    mat = np.eye(48, dtype=complex)
+    u, s, svh, _ = engine.split_svd(mat, 1e-7)
+    assert len(s) == 48 and np.allclose(s, 1.0)
+    assert np.abs(u @ svh - mat).max() < 1e-13
+
+
+def _engine_pt(bath, params, t_end, rho0, system):
+    pt = oqupy.pt_tempo_compute(bath, 0.0, t_end, params)
+    dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                 initial_state=rho0)
+    return pt, dyn
+
+
+@pytest.mark.parametrize("name", ["rho_A", "rho_C", "rho_E"])
+def test_pt_tempo_golden_parity(name):
+    _announce(f"pt_tempo {name}")
+    system, bath, params, rho0, n, rho, dec = models.golden_model(name)
+    t_end = n * params.dt + 1e-9 * params.dt
+    pt, dyn = _engine_pt(bath, params, t_end, rho0, system)
+    assert len(pt) == n
+    ref_states, ref_bonds, _ = models.oracle_pt_dynamics(system, bath, params,
+                                                         rho0, n)
+    assert list(pt.get_bond_dimensions()[1:-1]) == ref_bonds
+    assert np.abs(dyn.states - ref_states).max() < STATE_TOL
+    np.testing.assert_almost_equal(dyn.states[-1], rho, decimal=dec)
+
+
+def test_pt_tempo_growth_phase_parity():
+    """Finite memory (growth then end phase) on the headline bath."""
+    _announce("pt_tempo growth dkmax=6 N=24")
+    system, bath, params = models.spin_boson(0.5, 5.0, 0.2, 0.05, 6, 1e-7)
+    rho0 = op.spin_dm("z+")
+    n = 24
+    pt, dyn = _engine_pt(bath, params, n * 0.05 + 1e-11, rho0, system)
+    ref_states, ref_bonds, _ = models.oracle_pt_dynamics(system, bath, params,
+                                                         rho0, n)
+    assert list(pt.get_bond_dimensions()[1:-1]) == ref_bonds
+    sz = models.sigma_z_expectation(dyn.states)
+    sz_ref = models.sigma_z_expectation(ref_states)
+    assert np.abs(sz - sz_ref).max() < STATE_TOL
+    assert np.abs(dyn.states - ref_states).max() < STATE_TOL
+
+
+def test_pt_tempo_non_diagonal_coupling():
+    """A coupling sigma_x/2 exercises the basis transforms of the PT."""
+    _announce("pt_tempo non-diagonal")
+    corr = oqupy.PowerLawSD(0.3, 1.0, 5.0, temperature=0.0)
+    bath = oqupy.Bath(0.5 * op.sigma("x"), corr)
+    system = oqupy.System(0.5 * op.sigma("z"))
+    params = oqupy.TempoParameters(dt=0.1, dkmax=5, epsrel=1e-6)
+    rho0 = op.spin_dm("x+")
+    n = 10
+    pt, dyn = _engine_pt(bath, params, n * 0.1 + 1e-11, rho0, system)
+    from oracle.dynamics import compute_dynamics
+    from oracle.pt_tempo import PtTempoOracle, compute_caps
+    infl = models.CachedInfluence(bath, params)
+    ref = PtTempoOracle(2, infl, np.ones(4), n, 5, 1e-6)
+    ref.compute()
+    tens = ref.mpo_tensors()
+    caps = compute_caps(tens, 2)
+    props = system.get_propagators(0.1, 0.0, None, None)
+    states = compute_dynamics(props, rho0, n, [tens], [caps],
+                              transforms=[(pt.transform_in,
+                                           pt.transform_out)])
+    assert np.abs(dyn.states - np.array(states)).max() < STATE_TOL
+
+
+def test_compute_dynamics_host_tensors():
+    """compute_dynamics on a process tensor built by hand (host arrays)."""
+    _announce("dynamics host tensors")
+    system, bath, params = models.spin_boson(0.3, 3.0, 0.1, 0.05, 5, 1e-7)
+    rho0 = op.spin_dm("z+")
+    n = 15
+    ref_states, _, ref = models.oracle_pt_dynamics(system, bath, params, rho0,
+                                                   n)
+    from oracle.pt_tempo import compute_caps
+    tens = ref.mpo_tensors()
+    caps = compute_caps(tens, 2)
+    pt = oqupy.SimpleProcessTensor(2, dt=0.05)
+    for k, t in enumerate(tens):
+        pt.set_mpo_tensor(k, t)
+    for k, c in enumerate(caps):
+        pt.set_cap_tensor(k, c)
+    dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                 initial_state=rho0)
+    assert np.abs(dyn.states - ref_states).max() < STATE_TOL
+    # a prefix of the process tensor
+    dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                 initial_state=rho0, num_steps=7)
+    from oracle.dynamics import compute_dynamics
+    props = system.get_propagators(0.05, 0.0, None, None)
+    pre = compute_dynamics(props, rho0, 7, [tens], [caps])
+    assert np.abs(dyn.states - np.array(pre)).max() < STATE_TOL
+
+
+@pytest.mark.parametrize("name", ["rho_A", "rho_C", "rho_D", "rho_E"])
+def test_tempo_golden_parity(name):
+    _announce(f"tempo {name}")
+    system, bath, params, rho0, n, rho, dec = models.golden_model(name)
+    tempo = oqupy.Tempo(system, bath, params, rho0, start_time=0.0)
+    dyn = tempo.compute(end_time=n * params.dt + 1e-9 * params.dt)
+    ref_states, _ = models.oracle_tempo(system, bath, params, rho0, n)
+    assert len(dyn.times) == n + 1
+    assert np.abs(dyn.states - ref_states).max() < STATE_TOL
+    np.testing.assert_almost_equal(dyn.states[-1], rho, decimal=dec)
+
+
+def test_tempo_memory_cut_parity():
+    """TEMPO with a finite memory (window full, oldest site traced out)."""
+    _announce("tempo dkmax=8 N=30")
+    system, bath, params = models.spin_boson(0.3, 3.0, 0.0, 0.05, 8, 1e-6)
+    rho0 = op.spin_dm("z+")
+    tempo = oqupy.Tempo(system, bath, params, rho0, start_time=0.0)
+    dyn = tempo.compute(end_time=30 * 0.05 + 1e-11)
+    ref_states, _ = models.oracle_tempo(system, bath, params, rho0, 30)
+    assert np.abs(dyn.states - ref_states).max() < STATE_TOL
+
+
+def test_engine_errors_are_reported():
+    """Bad arguments come back as exceptions, not device faults."""
+    _announce("engine errors")
+    with pytest.raises(ValueError):
+        engine.split_svd(np.ones((3, 3)), -1.0)
+    with pytest.raises(RuntimeError):
+        bad = np.ones((4, 4)) * np.nan
+        engine.split_svd(bad, 1e-7)
