@@ -97,6 +97,14 @@ int pte_pt_get_cap(pte_pt_engine* e, int k, double* out, int cap, int* length);
 int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps,
                            int* max_bond, double* ms);
 
+/* Accumulated statistics of the Jacobi SVD kernel since creation: launches,
+ * device time (HIP events around every launch, milliseconds), algorithmic
+ * flops (4 (6 M N^2 + 20 N^3) per M x N SVD, N = min) and algorithmic HBM
+ * bytes (16 (2 M N + N^2)). No counterpart in the reference: it feeds the
+ * roofline report of the benchmark. */
+int pte_pt_svd_kernel_stats(pte_pt_engine* e, long* count, double* ms,
+                            double* flops, double* bytes);
+
 void pte_pt_destroy(pte_pt_engine* e);
 
 /* ---- TEMPO (oqupy/backends/tempo_backend.py:314-626) ---------------------- */

@@ -77,6 +77,8 @@ _SIGNATURES = {
     "pte_pt_get_cap": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int,
                                       _IP]),
     "pte_pt_last_step_stats": (ctypes.c_int, [_P, _IP, _IP, _IP, _DP]),
+    "pte_pt_svd_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_long),
+                                               _DP, _DP, _DP]),
     "pte_pt_destroy": (None, [_P]),
     "pte_tempo_create": (ctypes.c_int, [ctypes.POINTER(_TempoConfig),
                                         ctypes.POINTER(_P)]),
@@ -114,6 +116,9 @@ def load_library(path: Optional[str] = None):
         raise ImportError(
             f"The MI355X engine library {lib_path} is not built; run "
             "`python -m oqupy_amd.build` (hipcc, gfx950).")
+    # PyTorch-ROCm ships its own HIP runtime; loading it first makes the
+    # engine bind to that same runtime instead of starting a second one.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(lib_path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
@@ -142,6 +147,8 @@ def require_gpu() -> None:
     if not torch.cuda.is_available():
         raise RuntimeError("The MI355X engine needs a visible HIP device; "
                            "none is available.")
+    if not torch.cuda.is_initialized():
+        torch.cuda.init()
 
 
 def as_c128(array) -> np.ndarray:

@@ -111,6 +111,17 @@ class PtTempoBackend:
         self.step_stats.append(tuple(s.value for s in stats))
         return bool(more.value)
 
+    def svd_kernel_stats(self) -> dict:
+        """Accumulated launches, device milliseconds, algorithmic flops and
+        bytes of the Jacobi SVD kernel."""
+        count = ctypes.c_long()
+        vals = [ctypes.c_double() for _ in range(3)]
+        eng.check(self._lib.pte_pt_svd_kernel_stats(
+            self._handle, ctypes.byref(count),
+            *[ctypes.byref(v) for v in vals]))
+        return {"count": count.value, "ms": vals[0].value,
+                "flops": vals[1].value, "bytes": vals[2].value}
+
     def bond_dimensions(self) -> List[int]:
         """Bond dimensions of the device-resident MPS."""
         n = ctypes.c_int()

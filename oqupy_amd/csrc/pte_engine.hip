@@ -162,7 +162,7 @@ static std::vector<cplx> to_cplx(const double* data, long count) {
 // ---------------------------------------------------------------------------
 struct Workspace {
   hipStream_t stream = nullptr;
-  DevBuf G, X, V, carry, carry2, svh;
+  DevBuf G, X, V, carry, carry2, svh, gpart;
   double* sigma = nullptr;
   double* sorted = nullptr;
   int* perm = nullptr;
@@ -179,6 +179,10 @@ struct Workspace {
   // statistics of the current step
   int n_svd = 0, sweeps = 0, max_bond = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // accumulated timing of the Jacobi SVD kernel (HIP events on this stream)
+  hipEvent_t evj0 = nullptr, evj1 = nullptr;
+  long jac_count = 0;
+  double jac_ms = 0.0, jac_flops = 0.0, jac_bytes = 0.0;
 
   void init(double eps, int cap) {
     epsrel = eps;
@@ -205,17 +209,21 @@ struct Workspace {
     require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
     HIP_CHECK(hipEventCreate(&ev0));
     HIP_CHECK(hipEventCreate(&ev1));
+    HIP_CHECK(hipEventCreate(&evj0));
+    HIP_CHECK(hipEventCreate(&evj1));
   }
 
   void destroy() {
     if (!stream) return;
     G.release(stream); X.release(stream); V.release(stream);
     carry.release(stream); carry2.release(stream); svh.release(stream);
+    gpart.release(stream);
     (void)hipStreamSynchronize(stream);
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
     (void)hipFree(bar); (void)hipFree(flags); (void)hipFree(fro2); (void)hipFree(info_d);
     (void)hipHostFree(info_h);
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
+    (void)hipEventDestroy(evj0); (void)hipEventDestroy(evj1);
     (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -240,9 +248,17 @@ struct Workspace {
     int nb = (n + kJacobiBlock - 1) / kJacobiBlock;
     if (nb % 2) ++nb;
     if (nb < 2) nb = 2;
-    const int grid = nb / 2;
-    require(grid <= max_coop_blocks, PTE_ERR_CAPACITY,
+    const int npairs = nb / 2;
+    require(npairs <= max_coop_blocks, PTE_ERR_CAPACITY,
             "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
+    // row slices per block pair: only for tall enough matrices, so that each
+    // slice keeps at least 128 rows of X
+    int R = 1;
+    if (m >= 256) R = std::max(1, std::min({max_coop_blocks / npairs, m / 128, 16}));
+    const int grid = npairs * R;
+    auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
+    const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
+    if (R > 1) gpart.ensure((size_t)npairs * R * 1024, stream);
     HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned), stream));
     HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, stream));
     HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), stream));
@@ -250,14 +266,18 @@ struct Workspace {
     ja.X = X.p; ja.m = m; ja.n = n; ja.ldx = m;
     ja.V = V.p; ja.ldv = n;
     ja.nb = nb;
+    ja.slices = R; ja.xrows_per = xrows; ja.vrows_per = vrows;
+    ja.gpart = gpart.p;
     ja.tol = std::max(m, 16) * 2.220446049250313e-16;
     ja.abs_tol_scale = 2.220446049250313e-16;
     ja.max_sweeps = max_sweeps;
     ja.bar = bar; ja.flags = flags; ja.fro2 = fro2;
     ja.info = &info_d->sweeps;
     void* kargs[] = {&ja};
+    HIP_CHECK(hipEventRecord(evj0, stream));
     HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
                                          dim3(grid), dim3(256), kargs, 0, stream));
+    HIP_CHECK(hipEventRecord(evj1, stream));
     column_norm_kernel<<<n, 256, 0, stream>>>(X.p, m, n, m, sigma);
     rank_sort_kernel<<<(n + 255) / 256, 256, 0, stream>>>(sigma, n, perm, sorted);
     truncate_kernel<<<1, 1024, 0, stream>>>(sorted, n, epsrel, info_d, chi_cap);
@@ -272,6 +292,18 @@ struct Workspace {
             "bond dimension " + std::to_string(info_h->rank) +
                 " exceeds the capacity chi_cap=" + std::to_string(chi_cap));
     require(info_h->rank >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
+    {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
+      // algorithmic work of a thin complex SVD of an M x N matrix (R-SVD
+      // count of Golub & Van Loan, times 4 for complex arithmetic) and the
+      // HBM bytes of reading and writing M once plus writing V
+      const double mm = std::max(m, n), nn = std::min(m, n);
+      jac_ms += ms;
+      jac_count += 1;
+      jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
+      jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
+    }
     n_svd += 1;
     sweeps += info_h->sweeps;
     max_bond = std::max(max_bond, info_h->rank);
@@ -1259,6 +1291,17 @@ int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps, int* max_b
     if (sweeps) *sweeps = e->impl.ws.sweeps;
     if (max_bond) *max_bond = e->impl.ws.max_bond;
     if (ms) *ms = e->impl.last_ms;
+  });
+}
+
+int pte_pt_svd_kernel_stats(pte_pt_engine* e, long* count, double* ms, double* flops,
+                            double* bytes) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    if (count) *count = e->impl.ws.jac_count;
+    if (ms) *ms = e->impl.ws.jac_ms;
+    if (flops) *flops = e->impl.ws.jac_flops;
+    if (bytes) *bytes = e->impl.ws.jac_bytes;
   });
 }
 

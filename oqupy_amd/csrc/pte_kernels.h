@@ -42,6 +42,10 @@ struct JacobiArgs {
   double2* X; int m, n; long ldx;
   double2* V; long ldv;
   int nb;
+  int slices;         // workgroups per block pair (row slices)
+  int xrows_per;      // rows of X per slice (multiple of 32)
+  int vrows_per;      // rows of V per slice (multiple of 32)
+  double2* gpart;     // partial Grams [pair][slice][32*32] when slices > 1
   double tol, abs_tol_scale;
   int max_sweeps;
   unsigned* bar;

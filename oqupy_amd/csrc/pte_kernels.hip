@@ -178,18 +178,21 @@ mpo_apply_right_kernel(MpoApplyArgs a) {
 // Blocked one-sided Jacobi SVD.
 // X (m x n, column-major, leading dimension ldx) is orthogonalised column-wise:
 // X <- X Q with Q unitary accumulated in V (n x n, column-major). Columns are
-// grouped into blocks of JB; in every round each workgroup takes one block pair
-// (round-robin tournament over nb blocks), forms the 2JB x 2JB Gram matrix,
-// diagonalises it with cyclic two-sided Jacobi in LDS and applies the unitary
-// to the pair's columns of X and V. A sweep is nb-1 rounds; the iteration stops
-// after a sweep in which no pair needed a rotation. A pair needs one when some
+// grouped into blocks of JB; in every round each block pair of the
+// round-robin tournament over nb blocks is handled by `slices` workgroups,
+// each owning a slice of the rows of X and of V. The pair's 2JB x 2JB Gram
+// matrix is summed from the slices' partial Grams, diagonalised (cyclic
+// two-sided Jacobi in LDS, at most INNER_SWEEPS sweeps, identically in every
+// workgroup of the pair) and the resulting unitary applied to the slice's rows
+// of X and V. A sweep is nb-1 rounds; the iteration stops after a sweep in
+// which no pair needed a rotation. A pair needs one when some
 // |g_ij| > tol*sqrt(g_ii g_jj) and |g_ij| > abs_tol*sqrt(max(g_ii, g_jj)).
 // ---------------------------------------------------------------------------
 constexpr int JB = 16;          // block width
 constexpr int JP = 2 * JB;      // columns per block pair
 constexpr int JR = 32;          // rows per streamed chunk
 constexpr int JT = 256;         // threads per workgroup
-constexpr int INNER_SWEEPS = 4;
+constexpr int INNER_SWEEPS = 2;
 
 __device__ __forceinline__ void tournament_pair(int nplayers, int round, int slot,
                                                 int& a, int& b) {
@@ -202,19 +205,46 @@ __device__ __forceinline__ void tournament_pair(int nplayers, int round, int slo
   }
 }
 
+__device__ __forceinline__ void ordered_pair(int sub, int slot, int& p, int& q) {
+  tournament_pair(JP, sub, slot, p, q);
+  if (p > q) { const int t = p; p = q; q = t; }
+}
+
+// Rotation of the Hermitian 2x2 block [[app, apq], [conj(apq), aqq]]:
+// J = [[cs, sn], [-sn conj(ph), cs conj(ph)]], ph = apq / |apq|, chosen so that
+// J^H [[app, apq], [conj(apq), aqq]] J is diagonal (Rutishauser's formulas).
+__device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 apq,
+                                                double tol, double abs_tol, double* r) {
+  const double mag = sqrt(cabs2(apq));
+  r[0] = 1.0; r[1] = 0.0; r[2] = 1.0; r[3] = 0.0;
+  if (!(mag > tol * sqrt(fabs(app * aqq)) &&
+        mag > abs_tol * sqrt(fmax(fmax(app, aqq), 0.0)) && mag > 0.0))
+    return false;
+  const double zeta = (aqq - app) / (2.0 * mag);
+  const double t = (zeta == 0.0)
+      ? 1.0 : copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double cs = 1.0 / sqrt(1.0 + t * t);
+  r[0] = cs; r[1] = cs * t; r[2] = apq.x / mag; r[3] = apq.y / mag;
+  return true;
+}
+
 __global__ void __launch_bounds__(JT)
 jacobi_kernel(JacobiArgs a) {
   __shared__ double2 Xs[JR][JP + 1];
   __shared__ double2 Gs[JP][JP + 1];
   __shared__ double2 Ws[JP][JP + 1];
-  __shared__ double rot[JB][4];     // cs, sn, phase_re, phase_im
+  __shared__ double rot[JB][4];
   __shared__ int rotflag[JB];
   __shared__ int cols[JP];
-  __shared__ int need;
+  __shared__ int need, anyrot;
   __shared__ double red[JT];
 
   const int tid = threadIdx.x;
   const unsigned nblk = gridDim.x;
+  const int R = a.slices;
+  const int pair = blockIdx.x / R, slice = blockIdx.x % R;
+  const int x_lo = min(a.m, slice * a.xrows_per), x_hi = min(a.m, x_lo + a.xrows_per);
+  const int v_lo = min(a.n, slice * a.vrows_per), v_hi = min(a.n, v_lo + a.vrows_per);
   unsigned epoch = 0;
 
   // V = identity, Frobenius norm of X
@@ -241,55 +271,74 @@ jacobi_kernel(JacobiArgs a) {
                                             __HIP_MEMORY_SCOPE_AGENT));
   const double abs_tol = a.abs_tol_scale * fro;
   const double tol = a.tol;
+  const int ti = tid >> 4, tj = tid & 15;        // 2x2 Gram blocking
 
   int sweep = 0;
   bool converged = false;
   for (; sweep < a.max_sweeps; ++sweep) {
     for (int round = 0; round < a.nb - 1; ++round) {
       int bi, bj;
-      tournament_pair(a.nb, round, blockIdx.x, bi, bj);
+      tournament_pair(a.nb, round, pair, bi, bj);
       if (tid < JP) {
         const int c = (tid < JB) ? bi * JB + tid : bj * JB + (tid - JB);
         cols[tid] = (c < a.n) ? c : -1;
       }
       __syncthreads();
-      // ---- Gram matrix of the pair's columns ----
-      double2 g[4];
+      // ---- partial Gram of the pair's columns over this slice's rows ----
+      double2 g[2][2];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) g[q] = make_double2(0.0, 0.0);
-      for (int r0 = 0; r0 < a.m; r0 += JR) {
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) g[u][v] = make_double2(0.0, 0.0);
+      for (int r0 = x_lo; r0 < x_hi; r0 += JR) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int e = tid + JT * q;      // JR*JP = 1024 entries
+          const int e = tid + JT * q;
           const int rr = e % JR, cc = e / JR;
           const int gr = r0 + rr, gc = cols[cc];
-          Xs[rr][cc] = (gr < a.m && gc >= 0) ? a.X[gr + (long)gc * a.ldx]
-                                             : make_double2(0.0, 0.0);
+          Xs[rr][cc] = (gr < x_hi && gc >= 0) ? a.X[gr + (long)gc * a.ldx]
+                                              : make_double2(0.0, 0.0);
         }
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = tid + JT * q;      // JP*JP = 1024 Gram entries
-          const int gi = e / JP, gj = e % JP;
-#pragma unroll 8
-          for (int rr = 0; rr < JR; ++rr) cfmac(g[q], Xs[rr][gi], Xs[rr][gj]);
+#pragma unroll 4
+        for (int rr = 0; rr < JR; ++rr) {
+          const double2 a0 = Xs[rr][ti], a1 = Xs[rr][ti + 16];
+          const double2 b0 = Xs[rr][tj], b1 = Xs[rr][tj + 16];
+          cfmac(g[0][0], a0, b0);
+          cfmac(g[0][1], a0, b1);
+          cfmac(g[1][0], a1, b0);
+          cfmac(g[1][1], a1, b1);
         }
         __syncthreads();
       }
+      if (R == 1) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = tid + JT * q;
-        Gs[e / JP][e % JP] = g[q];
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 2; ++v) Gs[ti + 16 * u][tj + 16 * v] = g[u][v];
+      } else {
+        double2* mine = a.gpart + ((long)pair * R + slice) * (JP * JP);
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int v = 0; v < 2; ++v) mine[(ti + 16 * u) * JP + tj + 16 * v] = g[u][v];
+        grid_barrier(a.bar, nblk, epoch);
+        // identical summation order in every workgroup of the pair
+        for (int e = tid; e < JP * JP; e += JT) {
+          double2 acc = make_double2(0.0, 0.0);
+          const double2* base = a.gpart + (long)pair * R * (JP * JP) + e;
+          for (int sl = 0; sl < R; ++sl) acc = cadd(acc, base[(long)sl * (JP * JP)]);
+          Gs[e / JP][e % JP] = acc;
+        }
+      }
+      for (int e = tid; e < JP * JP; e += JT)
         Ws[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
-      }
       if (tid == 0) need = 0;
       __syncthreads();
       // ---- does the pair need a rotation? ----
       {
         int mine = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = tid + JT * q;
+        for (int e = tid; e < JP * JP; e += JT) {
           const int gi = e / JP, gj = e % JP;
           if (gi < gj && cols[gi] >= 0 && cols[gj] >= 0) {
             const double dii = Gs[gi][gi].x, djj = Gs[gj][gj].x;
@@ -305,118 +354,108 @@ jacobi_kernel(JacobiArgs a) {
       if (need) {
         // ---- cyclic two-sided Jacobi on Gs, accumulating Ws ----
         for (int isw = 0; isw < INNER_SWEEPS; ++isw) {
-          int any = 0;
+          if (tid == 0) anyrot = 0;
           for (int sub = 0; sub < JP - 1; ++sub) {
             if (tid < JB) {
               int p, q;
-              tournament_pair(JP, sub, tid, p, q);
-              if (p > q) { const int t = p; p = q; q = t; }
-              double cs = 1.0, sn = 0.0, phr = 1.0, phi = 0.0;
-              int flag = 0;
-              if (cols[p] >= 0 && cols[q] >= 0) {
-                const double app = Gs[p][p].x, aqq = Gs[q][q].x;
-                const double2 apq = Gs[p][q];
-                const double mag = sqrt(cabs2(apq));
-                if (mag > tol * sqrt(fabs(app * aqq)) &&
-                    mag > abs_tol * sqrt(fmax(fmax(app, aqq), 0.0)) && mag > 0.0) {
-                  phr = apq.x / mag;
-                  phi = apq.y / mag;
-                  const double zeta = (aqq - app) / (2.0 * mag);
-                  const double t = (zeta == 0.0)
-                      ? 1.0
-                      : copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                  cs = 1.0 / sqrt(1.0 + t * t);
-                  sn = cs * t;
-                  flag = 1;
+              ordered_pair(sub, tid, p, q);
+              double r[4];
+              const bool on = (cols[p] >= 0 && cols[q] >= 0) &&
+                  jacobi_rotation(Gs[p][p].x, Gs[q][q].x, Gs[p][q], tol, abs_tol, r);
+              rot[tid][0] = r[0]; rot[tid][1] = r[1]; rot[tid][2] = r[2]; rot[tid][3] = r[3];
+              rotflag[tid] = on;
+              if (on) anyrot = 1;
+            }
+            __syncthreads();
+            {
+              // block (P1, P2): G[(p1,q1),(p2,q2)] <- J1^H G J2
+              const int P1 = tid >> 4, P2 = tid & 15;
+              if (rotflag[P1] || rotflag[P2]) {
+                int p1, q1, p2, q2;
+                ordered_pair(sub, P1, p1, q1);
+                ordered_pair(sub, P2, p2, q2);
+                double2 b00 = Gs[p1][p2], b01 = Gs[p1][q2];
+                double2 b10 = Gs[q1][p2], b11 = Gs[q1][q2];
+                if (rotflag[P2]) {
+                  const double c = rot[P2][0], sn = rot[P2][1];
+                  const double2 cph = make_double2(rot[P2][2], -rot[P2][3]);
+                  const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
+                  const double2 n00 = make_double2(c * b00.x - sn * x01.x, c * b00.y - sn * x01.y);
+                  const double2 n01 = make_double2(sn * b00.x + c * x01.x, sn * b00.y + c * x01.y);
+                  const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
+                  const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
+                  b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+                }
+                if (rotflag[P1]) {
+                  const double c = rot[P1][0], sn = rot[P1][1];
+                  const double2 ph = make_double2(rot[P1][2], rot[P1][3]);
+                  const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
+                  const double2 n00 = make_double2(c * b00.x - sn * y10.x, c * b00.y - sn * y10.y);
+                  const double2 n01 = make_double2(c * b01.x - sn * y11.x, c * b01.y - sn * y11.y);
+                  const double2 n10 = make_double2(sn * b00.x + c * y10.x, sn * b00.y + c * y10.y);
+                  const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
+                  b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+                }
+                if (P1 == P2) {
+                  b01 = make_double2(0.0, 0.0);
+                  b10 = make_double2(0.0, 0.0);
+                  b00.y = 0.0;
+                  b11.y = 0.0;
+                }
+                Gs[p1][p2] = b00; Gs[p1][q2] = b01; Gs[q1][p2] = b10; Gs[q1][q2] = b11;
+              }
+              // W[:, (p,q)] <- W[:, (p,q)] J for two rows per thread
+              const int pr = tid & 15;
+              if (rotflag[pr]) {
+                int p, q;
+                ordered_pair(sub, pr, p, q);
+                const double c = rot[pr][0], sn = rot[pr][1];
+                const double2 cph = make_double2(rot[pr][2], -rot[pr][3]);
+                for (int row = tid >> 4; row < JP; row += 16) {
+                  const double2 wp = Ws[row][p];
+                  const double2 wq = cmul(Ws[row][q], cph);
+                  Ws[row][p] = make_double2(c * wp.x - sn * wq.x, c * wp.y - sn * wq.y);
+                  Ws[row][q] = make_double2(sn * wp.x + c * wq.x, sn * wp.y + c * wq.y);
                 }
               }
-              rot[tid][0] = cs; rot[tid][1] = sn; rot[tid][2] = phr; rot[tid][3] = phi;
-              rotflag[tid] = flag;
-            }
-            __syncthreads();
-            // J = [[cs, sn], [-sn*conj(ph), cs*conj(ph)]]; columns: (Gp,Gq) <- (Gp,Gq) J
-            for (int task = tid; task < JB * JP * 2; task += JT) {
-              const int which = task / (JB * JP);        // 0: G, 1: W
-              const int rem = task % (JB * JP);
-              const int pr = rem / JP, row = rem % JP;
-              if (!rotflag[pr]) continue;
-              int p, q;
-              tournament_pair(JP, sub, pr, p, q);
-              if (p > q) { const int t = p; p = q; q = t; }
-              const double cs = rot[pr][0], sn = rot[pr][1];
-              const double2 cph = make_double2(rot[pr][2], -rot[pr][3]);  // conj(ph)
-              double2 (*M)[JP + 1] = which ? Ws : Gs;
-              const double2 xp = M[row][p], xq = M[row][q];
-              const double2 xqc = cmul(xq, cph);
-              M[row][p] = make_double2(cs * xp.x - sn * xqc.x, cs * xp.y - sn * xqc.y);
-              M[row][q] = make_double2(sn * xp.x + cs * xqc.x, sn * xp.y + cs * xqc.y);
-            }
-            __syncthreads();
-            // rows: (Gp;Gq) <- J^H (Gp;Gq)
-            for (int task = tid; task < JB * JP; task += JT) {
-              const int pr = task / JP, col = task % JP;
-              if (!rotflag[pr]) continue;
-              int p, q;
-              tournament_pair(JP, sub, pr, p, q);
-              if (p > q) { const int t = p; p = q; q = t; }
-              const double cs = rot[pr][0], sn = rot[pr][1];
-              const double2 ph = make_double2(rot[pr][2], rot[pr][3]);
-              const double2 yp = Gs[p][col], yq = Gs[q][col];
-              const double2 yqp = cmul(ph, yq);   // conj(conj(ph)) * yq
-              Gs[p][col] = make_double2(cs * yp.x - sn * yqp.x, cs * yp.y - sn * yqp.y);
-              Gs[q][col] = make_double2(sn * yp.x + cs * yqp.x, sn * yp.y + cs * yqp.y);
-            }
-            __syncthreads();
-            if (tid < JB && rotflag[tid]) {
-              int p, q;
-              tournament_pair(JP, sub, tid, p, q);
-              Gs[p][q] = make_double2(0.0, 0.0);
-              Gs[q][p] = make_double2(0.0, 0.0);
-              Gs[p][p].y = 0.0;
-              Gs[q][q].y = 0.0;
-              any = 1;
             }
             __syncthreads();
           }
-          if (tid < JB) rotflag[tid] = any;   // reuse as a sweep flag
-          __syncthreads();
-          int anyrot = 0;
-          for (int i = 0; i < JB; ++i) anyrot |= rotflag[i];
-          __syncthreads();
           if (!anyrot) break;
         }
-        // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws ----
+        // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws over this slice ----
+        const int rr = tid >> 3, c0 = (tid & 7) * 4;
         for (int mat = 0; mat < 2; ++mat) {
           double2* base = mat ? a.V : a.X;
           const long ld = mat ? a.ldv : a.ldx;
-          const int rows = mat ? a.n : a.m;
-          for (int r0 = 0; r0 < rows; r0 += JR) {
+          const int lo = mat ? v_lo : x_lo, hi = mat ? v_hi : x_hi;
+          for (int r0 = lo; r0 < hi; r0 += JR) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
               const int e = tid + JT * q;
-              const int rr = e % JR, cc = e / JR;
-              const int gr = r0 + rr, gc = cols[cc];
-              Xs[rr][cc] = (gr < rows && gc >= 0) ? base[gr + (long)gc * ld]
-                                                  : make_double2(0.0, 0.0);
+              const int r_ = e % JR, cc = e / JR;
+              const int gr = r0 + r_, gc = cols[cc];
+              Xs[r_][cc] = (gr < hi && gc >= 0) ? base[gr + (long)gc * ld]
+                                                : make_double2(0.0, 0.0);
             }
             __syncthreads();
-            double2 outv[4];
+            double2 o[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int e = tid + JT * q;
-              const int rr = e % JR, cc = e / JR;
-              double2 acc = make_double2(0.0, 0.0);
+            for (int c = 0; c < 4; ++c) o[c] = make_double2(0.0, 0.0);
 #pragma unroll 8
-              for (int i = 0; i < JP; ++i) cfma(acc, Xs[rr][i], Ws[i][cc]);
-              outv[q] = acc;
+            for (int i = 0; i < JP; ++i) {
+              const double2 xv = Xs[rr][i];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) cfma(o[c], xv, Ws[i][c0 + c]);
             }
             __syncthreads();
+            const int gr = r0 + rr;
+            if (gr < hi) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int e = tid + JT * q;
-              const int rr = e % JR, cc = e / JR;
-              const int gr = r0 + rr, gc = cols[cc];
-              if (gr < rows && gc >= 0) base[gr + (long)gc * ld] = outv[q];
+              for (int c = 0; c < 4; ++c) {
+                const int gc = cols[c0 + c];
+                if (gc >= 0) base[gr + (long)gc * ld] = o[c];
+              }
             }
           }
         }
