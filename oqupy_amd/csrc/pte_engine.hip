@@ -160,9 +160,14 @@ static std::vector<cplx> to_cplx(const double* data, long count) {
 // ---------------------------------------------------------------------------
 // workspace and the truncated SVD
 // ---------------------------------------------------------------------------
+// Longest column for the QR-preconditioned SVD: apply_q_kernel holds one
+// column of Q in LDS (16 bytes per row, below 64 KiB with its 4 KiB of
+// reduction buffers).
+constexpr int kQrMaxRows = 3584;
+
 struct Workspace {
   hipStream_t stream = nullptr;
-  DevBuf G, X, V, carry, carry2, svh, gpart;
+  DevBuf G, X, V, carry, carry2, svh, gpart, Q, tau;
   double* sigma = nullptr;
   double* sorted = nullptr;
   int* perm = nullptr;
@@ -174,6 +179,7 @@ struct Workspace {
   SvdInfo* info_h = nullptr;
   int max_sweeps = 80;
   int max_coop_blocks = 0;
+  int qr_m = 0;
   double epsrel = 1e-7;
   int chi_cap = 512;
   // statistics of the current step
@@ -217,7 +223,7 @@ struct Workspace {
     if (!stream) return;
     G.release(stream); X.release(stream); V.release(stream);
     carry.release(stream); carry2.release(stream); svh.release(stream);
-    gpart.release(stream);
+    gpart.release(stream); Q.release(stream); tau.release(stream);
     (void)hipStreamSynchronize(stream);
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
     (void)hipFree(bar); (void)hipFree(flags); (void)hipFree(fro2); (void)hipFree(info_d);
@@ -310,6 +316,37 @@ struct Workspace {
     return info_h->rank;
   }
 
+  // Truncated SVD of a tall M (m x n, m >= n, column-major in Q, ld = m):
+  // Householder QR, then the Jacobi SVD of R^H (n x n, in X). Returns k;
+  // left singular vectors of M are Q W[:, perm] (see emit_left_q), and
+  // S V^H row j is conj(Z[:, perm[j]]) with Z the orthogonalised X.
+  int svd_tall_qr(int m, int n) {
+    require(m >= n && n >= 1, PTE_ERR_ARG, "internal: QR needs a tall matrix");
+    tau.ensure(n, stream);
+    const int grid = std::max(1, std::min({n, 256, max_coop_blocks}));
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned), stream));
+    QrArgs qa;
+    qa.A = Q.p; qa.m = m; qa.n = n; qa.lda = m; qa.tau = tau.p; qa.bar = bar;
+    void* kargs[] = {&qa};
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qr_kernel),
+                                         dim3(grid), dim3(256), kargs, 0, stream));
+    X.ensure((size_t)n * n, stream);
+    build_rh_kernel<<<grid_for((long)n * n), 256, 0, stream>>>(Q.p, m, n, X.p);
+    qr_m = m;
+    return svd(n, n);
+  }
+
+  // dst[j*dj + i*di] = (Q W[:, perm[j]])[i], i < qr_m, j < k
+  void emit_left_q(int n, long dj, long di, double2* dst, int k) {
+    ApplyQArgs aq;
+    aq.A = Q.p; aq.m = qr_m; aq.n = n; aq.lda = qr_m;
+    aq.tau = tau.p; aq.W = V.p; aq.ldw = n; aq.perm = perm; aq.info = info_d;
+    aq.dst = dst; aq.dj = dj; aq.di = di;
+    const size_t lds = (size_t)qr_m * sizeof(double2);
+    require(qr_m <= kQrMaxRows, PTE_ERR_CAPACITY, "internal: QR column too long for LDS");
+    apply_q_kernel<<<std::min(k, 1024), 256, lds, stream>>>(aq);
+  }
+
   void emit(const double2* src, long lds, int A, int B, long dj0, long djk,
             long da0, long dak, long db0, long dbk, int conj, double2* dst, int k) {
     EmitArgs ea;
@@ -355,14 +392,27 @@ struct Chain {
     for (int i = start; i < stop; ++i) {
       DTensor& A = sites[i];
       const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
-      // M = A as (Dl*P) x Dr; Jacobi on X = M^H (Dr x Dl*P), X[r + c*Dr] = conj(A[c*Dr + r])
       const long cnt = (long)Dl * P * Dr;
-      ws->X.ensure(cnt, st());
-      conj_copy_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, cnt, ws->X.p);
-      const int k = ws->svd(Dr, Dl * P);
-      // new site [l][p][j] = V[(l*P+p) + perm[j]*ldv]
-      DTensor U = dalloc(Dl, P, k, 1, st());
-      ws->emit(ws->V.p, Dl * P, Dl * P, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+      const int m = Dl * P;
+      int k;
+      DTensor U;
+      if (m >= Dr && m <= kQrMaxRows) {
+        // M = A as (Dl*P) x Dr: QR, then Jacobi on R^H (Dr x Dr)
+        ws->Q.ensure(cnt, st());
+        transpose_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, m, Dr, ws->Q.p);
+        k = ws->svd_tall_qr(m, Dr);
+        U = dalloc(Dl, P, k, 1, st());
+        ws->emit_left_q(Dr, /*dj*/ 1, /*di*/ k, U.p, k);
+      } else {
+        // M = A as (Dl*P) x Dr; Jacobi on X = M^H (Dr x Dl*P),
+        // X[r + c*Dr] = conj(A[c*Dr + r])
+        ws->X.ensure(cnt, st());
+        conj_copy_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, cnt, ws->X.p);
+        k = ws->svd(Dr, m);
+        // new site [l][p][j] = V[(l*P+p) + perm[j]*ldv]
+        U = dalloc(Dl, P, k, 1, st());
+        ws->emit(ws->V.p, m, m, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+      }
       // svh[j][r] = conj(Z[r + perm[j]*Dr])
       ws->svh.ensure((size_t)k * Dr, st());
       ws->emit(ws->X.p, Dr, Dr, 1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
@@ -384,15 +434,27 @@ struct Chain {
     for (int i = start; i > stop; --i) {
       DTensor& A = sites[i];
       const int Dl = A.d[0], P = A.d[1], Dr = A.d[2];
-      // M[(p,r), l] = A[l][p][r]; X = M^H (Dl x P*Dr): X[l + c*Dl] = conj(A[l*(P*Dr) + c])
-      ws->X.ensure((long)Dl * P * Dr, st());
-      conj_transpose_kernel<<<grid_for((long)Dl * P * Dr), 256, 0, st()>>>(
-          A.p, Dl, P * Dr, ws->X.p);
-      const int k = ws->svd(Dl, P * Dr);
-      // new site [j][p][r] = V[(p*Dr + r) + perm[j]*ldv]
-      DTensor U = dalloc(k, P, Dr, 1, st());
-      ws->emit(ws->V.p, (long)P * Dr, P * Dr, 1, /*dj*/ (long)P * Dr, 0, /*da*/ 1, 0, 0,
-               0, 0, U.p, k);
+      const long cnt = (long)Dl * P * Dr;
+      const int m = P * Dr;
+      int k;
+      DTensor U;
+      if (m >= Dl && m <= kQrMaxRows) {
+        // M[(p,r), l] = A[l][p][r] is column-major (P*Dr) x Dl in A's memory
+        ws->Q.ensure(cnt, st());
+        HIP_CHECK(hipMemcpyAsync(ws->Q.p, A.p, cnt * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, st()));
+        k = ws->svd_tall_qr(m, Dl);
+        U = dalloc(k, P, Dr, 1, st());
+        ws->emit_left_q(Dl, /*dj*/ m, /*di*/ 1, U.p, k);
+      } else {
+        // X = M^H (Dl x P*Dr): X[l + c*Dl] = conj(A[l*(P*Dr) + c])
+        ws->X.ensure(cnt, st());
+        conj_transpose_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, Dl, m, ws->X.p);
+        k = ws->svd(Dl, m);
+        // new site [j][p][r] = V[(p*Dr + r) + perm[j]*ldv]
+        U = dalloc(k, P, Dr, 1, st());
+        ws->emit(ws->V.p, m, m, 1, /*dj*/ m, 0, /*da*/ 1, 0, 0, 0, 0, U.p, k);
+      }
       // svh[j][l] = conj(Z[l + perm[j]*Dl])
       ws->svh.ensure((size_t)k * Dl, st());
       ws->emit(ws->X.p, Dl, Dl, 1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);

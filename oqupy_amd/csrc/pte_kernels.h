@@ -64,6 +64,24 @@ struct EmitArgs {
   double2* dst;
 };
 
+struct QrArgs {
+  double2* A; int m, n; long lda;
+  double2* tau;
+  unsigned* bar;
+};
+
+struct ApplyQArgs {
+  const double2* A; int m, n; long lda;   // reflectors from qr_kernel
+  const double2* tau;
+  const double2* W; long ldw;             // n x n unitary of the Jacobi
+  const int* perm;
+  const SvdInfo* info;
+  double2* dst; long dj, di;
+};
+
+__global__ void qr_kernel(QrArgs a);
+__global__ void build_rh_kernel(const double2* A, long lda, int n, double2* Y);
+__global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);
 __global__ void mpo_apply_right_kernel(MpoApplyArgs a);
@@ -75,6 +93,7 @@ __global__ void truncate_kernel(const double* sorted, int n, double epsrel, SvdI
 __global__ void emit_kernel(EmitArgs a);
 __global__ void conj_copy_kernel(const double2* src, long count, double2* dst);
 __global__ void conj_transpose_kernel(const double2* src, int R, int C, double2* dst);
+__global__ void transpose_kernel(const double2* src, int R, int C, double2* dst);
 __global__ void leg_vector_kernel(const double2* T, int A, int P, int B, const double2* vec,
                                   double2* out);
 __global__ void identity_carry_kernel(double2* dst, int R, int V);

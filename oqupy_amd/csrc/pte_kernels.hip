@@ -579,6 +579,162 @@ emit_kernel(EmitArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Householder QR of a tall matrix (preconditioner of the Jacobi SVD).
+// A (m x n, column-major, lda, m >= n) is overwritten by R (upper triangle)
+// and the reflectors v_j (below the diagonal, v_j[j] = 1 implicit), with
+// A = Q R, Q = H_0 H_1 ... H_{n-1}, H_j = I - tau_j v_j v_j^H (the LAPACK
+// zgeqr2/zlarfg convention). Column c belongs to workgroup c % gridDim.x;
+// at step j every workgroup applies H_j^H to its columns c > j, and the owner
+// of column j+1 then forms reflector j+1; one grid barrier per column.
+// ---------------------------------------------------------------------------
+__device__ double wg_sum(double v, double* red) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  red[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ void make_reflector(double2* A, int m, long lda, int j, double2* tau,
+                               double* red) {
+  double2* col = A + (long)j * lda;
+  double part = 0.0;
+  for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) part += cabs2(col[i]);
+  const double xnorm2 = wg_sum(part, red);
+  const double2 alpha = col[j];
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    if (threadIdx.x == 0) tau[j] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2),
+                                alpha.x);
+  // scale = 1 / (alpha - beta)
+  const double dr = alpha.x - beta, di = alpha.y;
+  const double den = dr * dr + di * di;
+  const double2 scale = make_double2(dr / den, -di / den);
+  for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) col[i] = cmul(col[i], scale);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tau[j] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    col[j] = make_double2(beta, 0.0);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+qr_kernel(QrArgs a) {
+  __shared__ double red[256];
+  __shared__ double redi[256];
+  const unsigned nblk = gridDim.x;
+  unsigned epoch = 0;
+  if (a.n > 0 && 0 % (int)nblk == (int)blockIdx.x) make_reflector(a.A, a.m, a.lda, 0, a.tau, red);
+  grid_barrier(a.bar, nblk, epoch);
+  for (int j = 0; j + 1 < a.n; ++j) {
+    const double2* v = a.A + (long)j * a.lda;
+    const double2 t = a.tau[j];
+    const double2 tc = make_double2(t.x, -t.y);   // apply H_j^H
+    int c0 = j + 1 + (((int)blockIdx.x - (j + 1)) % (int)nblk + (int)nblk) % (int)nblk;
+    for (int c = c0; c < a.n; c += nblk) {
+      double2* col = a.A + (long)c * a.lda;
+      if (t.x != 0.0 || t.y != 0.0) {
+        double pr = 0.0, pi = 0.0;
+        for (int i = j + threadIdx.x; i < a.m; i += blockDim.x) {
+          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
+          const double2 x = col[i];
+          pr += vi.x * x.x + vi.y * x.y;      // conj(v) * x
+          pi += vi.x * x.y - vi.y * x.x;
+        }
+        __syncthreads();
+        red[threadIdx.x] = pr;
+        redi[threadIdx.x] = pi;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+          if (threadIdx.x < s) {
+            red[threadIdx.x] += red[threadIdx.x + s];
+            redi[threadIdx.x] += redi[threadIdx.x + s];
+          }
+          __syncthreads();
+        }
+        const double2 w = make_double2(red[0], redi[0]);
+        const double2 tw = cmul(tc, w);
+        for (int i = j + threadIdx.x; i < a.m; i += blockDim.x) {
+          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
+          col[i] = csub(col[i], cmul(vi, tw));
+        }
+        __syncthreads();
+      }
+      if (c == j + 1) make_reflector(a.A, a.m, a.lda, j + 1, a.tau, red);
+    }
+    grid_barrier(a.bar, nblk, epoch);
+  }
+}
+
+// Y = R^H (n x n, column-major) from the upper triangle of A (lda)
+__global__ void __launch_bounds__(256)
+build_rh_kernel(const double2* A, long lda, int n, double2* Y) {
+  const long total = (long)n * n;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int i = e % n;
+    const int c = e / n;
+    Y[e] = (c <= i) ? cconj(A[c + (long)i * lda]) : make_double2(0.0, 0.0);
+  }
+}
+
+// Left singular vectors Q W[:, perm[j]] (j < k): one workgroup per column,
+// the column held in LDS, reflectors applied from the last to the first;
+// written to dst[j*dj + i*di] for i < m.
+__global__ void __launch_bounds__(256)
+apply_q_kernel(ApplyQArgs a) {
+  extern __shared__ double2 cvec[];
+  __shared__ double red[256];
+  __shared__ double redi[256];
+  const int k = a.info->rank;
+  for (int j = blockIdx.x; j < k; j += gridDim.x) {
+    const double2* w = a.W + (long)a.perm[j] * a.ldw;
+    for (int i = threadIdx.x; i < a.m; i += blockDim.x)
+      cvec[i] = (i < a.n) ? w[i] : make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int r = a.n - 1; r >= 0; --r) {
+      const double2 t = a.tau[r];
+      if (t.x == 0.0 && t.y == 0.0) continue;
+      const double2* v = a.A + (long)r * a.lda;
+      double pr = 0.0, pi = 0.0;
+      for (int i = r + threadIdx.x; i < a.m; i += blockDim.x) {
+        const double2 vi = (i == r) ? make_double2(1.0, 0.0) : v[i];
+        const double2 x = cvec[i];
+        pr += vi.x * x.x + vi.y * x.y;
+        pi += vi.x * x.y - vi.y * x.x;
+      }
+      red[threadIdx.x] = pr;
+      redi[threadIdx.x] = pi;
+      __syncthreads();
+      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+          red[threadIdx.x] += red[threadIdx.x + s];
+          redi[threadIdx.x] += redi[threadIdx.x + s];
+        }
+        __syncthreads();
+      }
+      const double2 tw = cmul(t, make_double2(red[0], redi[0]));
+      __syncthreads();
+      for (int i = r + threadIdx.x; i < a.m; i += blockDim.x) {
+        const double2 vi = (i == r) ? make_double2(1.0, 0.0) : v[i];
+        cvec[i] = csub(cvec[i], cmul(vi, tw));
+      }
+      __syncthreads();
+    }
+    for (int i = threadIdx.x; i < a.m; i += blockDim.x)
+      a.dst[(long)j * a.dj + (long)i * a.di] = cvec[i];
+    __syncthreads();
+  }
+}
+
 // X (rows x cols, column-major, ld = rows) = conj(src) with src row-major
 // (cols x rows): X[r + c*rows] = conj(src[c*rows + r]) -- a plain conjugate
 __global__ void __launch_bounds__(256)
@@ -595,6 +751,17 @@ conj_transpose_kernel(const double2* src, int R, int C, double2* dst) {
     const int r = e % R;
     const long c = e / R;
     dst[e] = cconj(src[(long)r * C + c]);
+  }
+}
+
+// X[r + c*R] = src[r*C + c]: row-major R x C to column-major
+__global__ void __launch_bounds__(256)
+transpose_kernel(const double2* src, int R, int C, double2* dst) {
+  const long total = (long)R * C;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int r = e % R;
+    const long c = e / R;
+    dst[e] = src[(long)r * C + c];
   }
 }
 
