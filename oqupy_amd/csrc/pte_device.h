@@ -41,24 +41,85 @@ __device__ __forceinline__ void cfmac(double2& acc, double2 a, double2 b) {
 }
 __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 
+// Sum over one 64-lane wavefront; every lane returns lane 0's total, so the
+// result is bit-identical across the wavefront.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
+// Sum over a 256-thread workgroup with one __syncthreads: wavefront sums go
+// to slot[parity][wave]; alternating the parity between calls keeps a fast
+// wavefront from overwriting slots that another is still reading. Every
+// thread returns the same value.
+__device__ __forceinline__ double block_sum256(double v, double (*slot)[4], int& parity) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) slot[parity][threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (slot[parity][0] + slot[parity][1]) + (slot[parity][2] + slot[parity][3]);
+  parity ^= 1;
+  return r;
+}
+
+__device__ __forceinline__ double2 block_sum256c(double2 v, double (*slot)[4],
+                                                 double (*sloti)[4], int& parity) {
+  v.x = wave_sum(v.x);
+  v.y = wave_sum(v.y);
+  if ((threadIdx.x & 63) == 0) {
+    slot[parity][threadIdx.x >> 6] = v.x;
+    sloti[parity][threadIdx.x >> 6] = v.y;
+  }
+  __syncthreads();
+  const double2 r = make_double2(
+      (slot[parity][0] + slot[parity][1]) + (slot[parity][2] + slot[parity][3]),
+      (sloti[parity][0] + sloti[parity][1]) + (sloti[parity][2] + sloti[parity][3]));
+  parity ^= 1;
+  return r;
+}
+
+// Agent-scope (device-coherent) accesses for data that workgroups of one
+// cooperative launch exchange. They bypass the per-XCD L2 copies, so the grid
+// barrier below needs no L2 writeback or invalidation; every read of such
+// data after a barrier and every write before one goes through these.
+__device__ __forceinline__ double ld_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_agent(const int* p) {
+  return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 ld_agent(const double2* p) {
+  return make_double2(ld_agent(&p->x), ld_agent(&p->y));
+}
+__device__ __forceinline__ void st_agent(double2* p, double2 v) {
+  st_agent(&p->x, v.x);
+  st_agent(&p->y, v.y);
+}
+
 // Grid-wide barrier for a cooperatively launched grid (all workgroups
-// co-resident). `counter` is zeroed before the launch and only grows; the
-// barrier of generation `epoch` waits until epoch * nblocks arrivals.
-// Agent-scope release/acquire fences make the data written before the
-// barrier visible to every XCD's L2 after it.
-__device__ __forceinline__ void grid_barrier(unsigned* counter, unsigned nblocks,
+// co-resident). bar[0] is zeroed before the launch and only grows; the
+// barrier of generation `epoch` waits until epoch * nblocks arrivals. Every
+// wavefront first waits for its outstanding memory operations (s_waitcnt 0),
+// so its agent-scope stores are performed before its workgroup arrives.
+constexpr unsigned kBarrierWords = 64;
+
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks,
                                              unsigned& epoch) {
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   epoch += 1;
   if (threadIdx.x == 0) {
-    __threadfence();
-    atomicAdd(counter, 1u);
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned target = epoch * nblocks;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE,
-                             __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-    }
-    __threadfence();
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
   }
   __syncthreads();
 }

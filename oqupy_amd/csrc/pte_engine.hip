@@ -17,6 +17,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <cstdio>
+#include <cstdlib>
 
 #include "ptengine.h"
 #include "pte_device.h"
@@ -165,9 +167,24 @@ static std::vector<cplx> to_cplx(const double* data, long count) {
 // reduction buffers).
 constexpr int kQrMaxRows = 3584;
 
+// largest rank a truncated SVD of an m x n matrix can keep
+static inline int k_bound(int m, int n) { return std::max(1, std::min(m, n)); }
+
 struct Workspace {
   hipStream_t stream = nullptr;
-  DevBuf G, X, V, carry, carry2, svh, gpart, Q, tau;
+  DevBuf G, X, V, carry, carry2, svh, gpart, Q;
+  // pivoted-QR state of the last svd_qrcp call
+  double2* qr_tau = nullptr;
+  double2* qr_vscale = nullptr;
+  double* qr_beta = nullptr;
+  int* qr_piv = nullptr;
+  int* qr_done = nullptr;
+  size_t qr_cap = 0;
+  QrcpCand* qr_cand = nullptr;
+  int* qr_rank_d = nullptr;
+  int* qr_rank_h = nullptr;
+  int qr_m = 0, qr_n = 0, qr_r = 0;
+  int num_cus = 0;
   double* sigma = nullptr;
   double* sorted = nullptr;
   int* perm = nullptr;
@@ -178,8 +195,10 @@ struct Workspace {
   SvdInfo* info_d = nullptr;
   SvdInfo* info_h = nullptr;
   int max_sweeps = 80;
+  int inner_sweeps = 1;      // tuning: PTE_JACOBI_INNER_SWEEPS
+  int qr_blocks = 256;       // tuning: PTE_QR_BLOCKS (workgroups of the pivoted QR)
+  int slice_rows = 64;       // tuning: PTE_JACOBI_SLICE_ROWS (min rows per slice)
   int max_coop_blocks = 0;
-  int qr_m = 0;
   double epsrel = 1e-7;
   int chi_cap = 512;
   // statistics of the current step
@@ -187,7 +206,14 @@ struct Workspace {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // accumulated timing of the Jacobi SVD kernel (HIP events on this stream)
   hipEvent_t evj0 = nullptr, evj1 = nullptr;
-  long jac_count = 0;
+  hipEvent_t evq0 = nullptr, evq1 = nullptr;
+  long jac_count = 0, qr_count = 0;
+  double qr_ms = 0.0;
+  // optional per-SVD log (environment variable PTE_SVD_TRACE = file path):
+  // m, n, pivoted-QR rank (0 without QR), kept rank, sweeps, QR ms, Jacobi ms
+  FILE* trace = nullptr;
+  int tr_m = 0, tr_n = 0, tr_r = 0;
+  float tr_qr_ms = 0.f;
   double jac_ms = 0.0, jac_flops = 0.0, jac_bytes = 0.0;
 
   void init(double eps, int cap) {
@@ -201,7 +227,7 @@ struct Workspace {
       uint64_t thr = ~0ull;
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
-    HIP_CHECK(hipMalloc((void**)&bar, sizeof(unsigned)));
+    HIP_CHECK(hipMalloc((void**)&bar, sizeof(unsigned) * kBarrierWords));
     HIP_CHECK(hipMalloc((void**)&flags, sizeof(int) * max_sweeps));
     HIP_CHECK(hipMalloc((void**)&fro2, sizeof(double)));
     HIP_CHECK(hipMalloc((void**)&info_d, sizeof(SvdInfo)));
@@ -212,24 +238,45 @@ struct Workspace {
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     max_coop_blocks = per_cu * prop.multiProcessorCount;
+    num_cus = prop.multiProcessorCount;
+    HIP_CHECK(hipMalloc((void**)&qr_cand, sizeof(QrcpCand) * 2 * 256));
+    HIP_CHECK(hipMalloc((void**)&qr_rank_d, sizeof(int)));
+    HIP_CHECK(hipHostMalloc((void**)&qr_rank_h, sizeof(int), hipHostMallocDefault));
     require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
     HIP_CHECK(hipEventCreate(&ev0));
     HIP_CHECK(hipEventCreate(&ev1));
     HIP_CHECK(hipEventCreate(&evj0));
     HIP_CHECK(hipEventCreate(&evj1));
+    HIP_CHECK(hipEventCreate(&evq0));
+    HIP_CHECK(hipEventCreate(&evq1));
+    if (const char* v = std::getenv("PTE_JACOBI_INNER_SWEEPS"))
+      inner_sweeps = std::max(1, std::min(8, std::atoi(v)));
+    if (const char* v = std::getenv("PTE_JACOBI_SLICE_ROWS"))
+      slice_rows = std::max(32, std::min(4096, std::atoi(v)));
+    if (const char* v = std::getenv("PTE_QR_BLOCKS"))
+      qr_blocks = std::max(1, std::min(256, std::atoi(v)));
+    if (const char* path = std::getenv("PTE_SVD_TRACE")) {
+      if (*path) trace = std::fopen(path, "a");
+    }
   }
 
   void destroy() {
     if (!stream) return;
     G.release(stream); X.release(stream); V.release(stream);
     carry.release(stream); carry2.release(stream); svh.release(stream);
-    gpart.release(stream); Q.release(stream); tau.release(stream);
+    gpart.release(stream); Q.release(stream);
     (void)hipStreamSynchronize(stream);
+    (void)hipFree(qr_tau); (void)hipFree(qr_vscale); (void)hipFree(qr_beta);
+    (void)hipFree(qr_piv); (void)hipFree(qr_done); (void)hipFree(qr_cand);
+    (void)hipFree(qr_rank_d); (void)hipHostFree(qr_rank_h);
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
     (void)hipFree(bar); (void)hipFree(flags); (void)hipFree(fro2); (void)hipFree(info_d);
     (void)hipHostFree(info_h);
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(evj0); (void)hipEventDestroy(evj1);
+    (void)hipEventDestroy(evq0); (void)hipEventDestroy(evq1);
+    if (trace) std::fclose(trace);
+    trace = nullptr;
     (void)hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -257,15 +304,16 @@ struct Workspace {
     const int npairs = nb / 2;
     require(npairs <= max_coop_blocks, PTE_ERR_CAPACITY,
             "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
-    // row slices per block pair: only for tall enough matrices, so that each
-    // slice keeps at least 128 rows of X
-    int R = 1;
-    if (m >= 256) R = std::max(1, std::min({max_coop_blocks / npairs, m / 128, 16}));
+    // row slices per block pair: each slice keeps at least slice_rows rows of
+    // X, at most 16 slices, within the co-resident grid and the barrier's
+    // 1024 workgroups
+    const int R = std::max(1, std::min({std::min(max_coop_blocks, 1024) / npairs,
+                                        (m + slice_rows - 1) / slice_rows, 16}));
     const int grid = npairs * R;
     auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
     const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
     if (R > 1) gpart.ensure((size_t)npairs * R * 1024, stream);
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned), stream));
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, stream));
     HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, stream));
     HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), stream));
     JacobiArgs ja;
@@ -277,6 +325,7 @@ struct Workspace {
     ja.tol = std::max(m, 16) * 2.220446049250313e-16;
     ja.abs_tol_scale = 2.220446049250313e-16;
     ja.max_sweeps = max_sweeps;
+    ja.inner_sweeps = inner_sweeps;
     ja.bar = bar; ja.flags = flags; ja.fro2 = fro2;
     ja.info = &info_d->sweeps;
     void* kargs[] = {&ja};
@@ -309,6 +358,12 @@ struct Workspace {
       jac_count += 1;
       jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
       jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
+      if (trace) {
+        if (tr_r == 0) { tr_m = n; tr_n = m; tr_qr_ms = 0.f; }
+        std::fprintf(trace, "%d,%d,%d,%d,%d,%.4f,%.4f\n", tr_m, tr_n, tr_r, info_h->rank,
+                     info_h->sweeps, tr_qr_ms, ms);
+        tr_r = 0;
+      }
     }
     n_svd += 1;
     sweeps += info_h->sweeps;
@@ -316,44 +371,94 @@ struct Workspace {
     return info_h->rank;
   }
 
-  // Truncated SVD of a tall M (m x n, m >= n, column-major in Q, ld = m):
-  // Householder QR, then the Jacobi SVD of R^H (n x n, in X). Returns k;
-  // left singular vectors of M are Q W[:, perm] (see emit_left_q), and
-  // S V^H row j is conj(Z[:, perm[j]]) with Z the orthogonalised X.
-  int svd_tall_qr(int m, int n) {
-    require(m >= n && n >= 1, PTE_ERR_ARG, "internal: QR needs a tall matrix");
-    tau.ensure(n, stream);
-    const int grid = std::max(1, std::min({n, 256, max_coop_blocks}));
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned), stream));
-    QrArgs qa;
-    qa.A = Q.p; qa.m = m; qa.n = n; qa.lda = m; qa.tau = tau.p; qa.bar = bar;
-    void* kargs[] = {&qa};
-    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qr_kernel),
-                                         dim3(grid), dim3(256), kargs, 0, stream));
-    X.ensure((size_t)n * n, stream);
-    build_rh_kernel<<<grid_for((long)n * n), 256, 0, stream>>>(Q.p, m, n, X.p);
-    qr_m = m;
-    return svd(n, n);
+  void ensure_qr(size_t n) {
+    if (n <= qr_cap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    (void)hipFree(qr_tau); (void)hipFree(qr_vscale); (void)hipFree(qr_beta);
+    (void)hipFree(qr_piv); (void)hipFree(qr_done);
+    const size_t c = std::max<size_t>(n, 1024);
+    HIP_CHECK(hipMalloc((void**)&qr_tau, c * sizeof(double2)));
+    HIP_CHECK(hipMalloc((void**)&qr_vscale, c * sizeof(double2)));
+    HIP_CHECK(hipMalloc((void**)&qr_beta, c * sizeof(double)));
+    HIP_CHECK(hipMalloc((void**)&qr_piv, c * sizeof(int)));
+    HIP_CHECK(hipMalloc((void**)&qr_done, c * sizeof(int)));
+    qr_cap = c;
   }
 
-  // dst[j*dj + i*di] = (Q W[:, perm[j]])[i], i < qr_m, j < k
-  void emit_left_q(int n, long dj, long di, double2* dst, int k) {
+  // Truncated SVD of M (m x n, column-major in Q, ld = m, m <= kQrMaxRows):
+  // pivoted QR  M P = Q R  (stopped once the unfactored block is below
+  // eps * ||M||_F, r steps), then the Jacobi SVD of Y = R^H (n x r) in X,
+  // Y W = Z. With perm the descending order of the column norms of Z:
+  //   left singular vector j = Q W[:, perm[j]]            (emit_left_q)
+  //   row j of S V^H at column piv[s] = conj(Z[s, perm[j]])  (emit_svh)
+  // Returns the kept rank k.
+  int svd_qrcp(int m, int n) {
+    require(m >= 1 && n >= 1 && m <= kQrMaxRows, PTE_ERR_ARG, "internal: bad QR shape");
+    ensure_qr((size_t)n);
+    const int nblk = std::max(1, std::min({n, qr_blocks, num_cus}));
+    require((n + nblk - 1) / nblk <= kQrcpMaxLocal, PTE_ERR_CAPACITY,
+            "matrix of " + std::to_string(n) + " columns exceeds the pivoted QR");
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, stream));
+    QrcpArgs qa;
+    qa.A = Q.p; qa.m = m; qa.n = n; qa.lda = m;
+    qa.drop = 2.220446049250313e-16;
+    qa.tau = qr_tau; qa.vscale = qr_vscale; qa.beta = qr_beta;
+    qa.piv = qr_piv; qa.done = qr_done; qa.cand = qr_cand; qa.rank = qr_rank_d;
+    qa.bar = bar;
+    void* kargs[] = {&qa};
+    HIP_CHECK(hipEventRecord(evq0, stream));
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qrcp_kernel),
+                                         dim3(nblk), dim3(256), kargs,
+                                         (size_t)m * sizeof(double2), stream));
+    HIP_CHECK(hipEventRecord(evq1, stream));
+    HIP_CHECK(hipMemcpyAsync(qr_rank_h, qr_rank_d, sizeof(int), hipMemcpyDeviceToHost,
+                             stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    const int r = *qr_rank_h;
+    require(r >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
+    {
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, evq0, evq1));
+      qr_ms += ms;
+      qr_count += 1;
+      tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = ms;
+    }
+    X.ensure((size_t)n * r, stream);
+    build_rh_kernel<<<grid_for((long)n * r), 256, 0, stream>>>(Q.p, m, n, qr_piv, qr_beta,
+                                                               qr_rank_d, X.p);
+    qr_m = m; qr_n = n; qr_r = r;
+    return svd(n, r);
+  }
+
+  // dst[j*dj + a*da + b*db] = (Q W[:, perm[j]])[a*B + b] for j < k, with
+  // every stride d = d0 + dk * k
+  void emit_left_q(int B, long dj0, long djk, long da0, long dak, long db0, long dbk,
+                   double2* dst, int k) {
     ApplyQArgs aq;
-    aq.A = Q.p; aq.m = qr_m; aq.n = n; aq.lda = qr_m;
-    aq.tau = tau.p; aq.W = V.p; aq.ldw = n; aq.perm = perm; aq.info = info_d;
-    aq.dst = dst; aq.dj = dj; aq.di = di;
-    const size_t lds = (size_t)qr_m * sizeof(double2);
-    require(qr_m <= kQrMaxRows, PTE_ERR_CAPACITY, "internal: QR column too long for LDS");
-    apply_q_kernel<<<std::min(k, 1024), 256, lds, stream>>>(aq);
+    aq.A = Q.p; aq.m = qr_m; aq.lda = qr_m;
+    aq.tau = qr_tau; aq.vscale = qr_vscale; aq.piv = qr_piv; aq.qrank = qr_rank_d;
+    aq.W = V.p; aq.ldw = qr_r; aq.perm = perm; aq.info = info_d;
+    aq.B = B;
+    aq.dj0 = dj0; aq.djk = djk; aq.da0 = da0; aq.dak = dak; aq.db0 = db0; aq.dbk = dbk;
+    aq.dst = dst;
+    apply_q_kernel<<<std::min(k, 1024), 256, (size_t)qr_m * sizeof(double2), stream>>>(aq);
+  }
+
+  // S V^H rows of the last svd_qrcp: column c = a*B + b of M goes to
+  // dst[j*dj + a*da + b*db], conjugated
+  void emit_svh(int B, long dj0, long djk, long da0, long dak, long db0, long dbk,
+                double2* dst, int k) {
+    emit(X.p, qr_n, qr_n / B, B, dj0, djk, da0, dak, db0, dbk, 1, dst, k, qr_piv);
   }
 
   void emit(const double2* src, long lds, int A, int B, long dj0, long djk,
-            long da0, long dak, long db0, long dbk, int conj, double2* dst, int k) {
+            long da0, long dak, long db0, long dbk, int conj, double2* dst, int k,
+            const int* rowmap = nullptr) {
     EmitArgs ea;
     ea.src = src; ea.lds = lds; ea.perm = perm; ea.info = info_d;
     ea.A = A; ea.B = B;
     ea.dj0 = dj0; ea.djk = djk; ea.da0 = da0; ea.dak = dak; ea.db0 = db0; ea.dbk = dbk;
-    ea.conj = conj; ea.dst = dst;
+    ea.conj = conj; ea.rowmap = rowmap; ea.dst = dst;
     emit_kernel<<<grid_for((long)A * B * k), 256, 0, stream>>>(ea);
   }
 
@@ -396,13 +501,17 @@ struct Chain {
       const int m = Dl * P;
       int k;
       DTensor U;
-      if (m >= Dr && m <= kQrMaxRows) {
-        // M = A as (Dl*P) x Dr: QR, then Jacobi on R^H (Dr x Dr)
+      ws->svh.ensure((size_t)k_bound(m, Dr) * Dr, st());
+      if (m <= kQrMaxRows) {
+        // M = A as (Dl*P) x Dr, column-major copy for the pivoted QR
         ws->Q.ensure(cnt, st());
         transpose_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, m, Dr, ws->Q.p);
-        k = ws->svd_tall_qr(m, Dr);
+        k = ws->svd_qrcp(m, Dr);
+        // new site [l][p][j]: row i = (l,p) -> i*k + j
         U = dalloc(Dl, P, k, 1, st());
-        ws->emit_left_q(Dr, /*dj*/ 1, /*di*/ k, U.p, k);
+        ws->emit_left_q(1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, U.p, k);
+        // svh[j][r]
+        ws->emit_svh(1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, ws->svh.p, k);
       } else {
         // M = A as (Dl*P) x Dr; Jacobi on X = M^H (Dr x Dl*P),
         // X[r + c*Dr] = conj(A[c*Dr + r])
@@ -412,10 +521,9 @@ struct Chain {
         // new site [l][p][j] = V[(l*P+p) + perm[j]*ldv]
         U = dalloc(Dl, P, k, 1, st());
         ws->emit(ws->V.p, m, m, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+        // svh[j][r] = conj(Z[r + perm[j]*Dr])
+        ws->emit(ws->X.p, Dr, Dr, 1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       }
-      // svh[j][r] = conj(Z[r + perm[j]*Dr])
-      ws->svh.ensure((size_t)k * Dr, st());
-      ws->emit(ws->X.p, Dr, Dr, 1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       // next site: [j][q][s] = sum_r svh[j][r] next[r][q][s]
       DTensor& Bn = sites[i + 1];
       require(Bn.d[0] == Dr, PTE_ERR_ARG, "internal: bond mismatch in sweep");
@@ -438,14 +546,18 @@ struct Chain {
       const int m = P * Dr;
       int k;
       DTensor U;
-      if (m >= Dl && m <= kQrMaxRows) {
+      ws->svh.ensure((size_t)k_bound(m, Dl) * Dl, st());
+      if (m <= kQrMaxRows) {
         // M[(p,r), l] = A[l][p][r] is column-major (P*Dr) x Dl in A's memory
         ws->Q.ensure(cnt, st());
         HIP_CHECK(hipMemcpyAsync(ws->Q.p, A.p, cnt * sizeof(double2),
                                  hipMemcpyDeviceToDevice, st()));
-        k = ws->svd_tall_qr(m, Dl);
+        k = ws->svd_qrcp(m, Dl);
+        // new site [j][p][r]: row i = (p,r) -> j*m + i
         U = dalloc(k, P, Dr, 1, st());
-        ws->emit_left_q(Dl, /*dj*/ m, /*di*/ 1, U.p, k);
+        ws->emit_left_q(1, /*dj*/ m, 0, /*da*/ 1, 0, 0, 0, U.p, k);
+        // svh[j][l]
+        ws->emit_svh(1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, ws->svh.p, k);
       } else {
         // X = M^H (Dl x P*Dr): X[l + c*Dl] = conj(A[l*(P*Dr) + c])
         ws->X.ensure(cnt, st());
@@ -454,10 +566,9 @@ struct Chain {
         // new site [j][p][r] = V[(p*Dr + r) + perm[j]*ldv]
         U = dalloc(k, P, Dr, 1, st());
         ws->emit(ws->V.p, m, m, 1, /*dj*/ m, 0, /*da*/ 1, 0, 0, 0, 0, U.p, k);
+        // svh[j][l] = conj(Z[l + perm[j]*Dl])
+        ws->emit(ws->X.p, Dl, Dl, 1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       }
-      // svh[j][l] = conj(Z[l + perm[j]*Dl])
-      ws->svh.ensure((size_t)k * Dl, st());
-      ws->emit(ws->X.p, Dl, Dl, 1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       // previous site: [l'][q][j] = sum_l prev[l'][q][l] svh[j][l]
       DTensor& Bp = sites[i - 1];
       require(Bp.d[2] == Dl, PTE_ERR_ARG, "internal: bond mismatch in sweep");
@@ -515,22 +626,39 @@ struct Chain {
         set_site(ia, T);
         break;
       }
-      // X = M^H with M[(x,s),(l,w)]: X is (Dl*Wl) x (Dx*S)
       const int mrows = Dl * Wl, ncols = Dx * S;
-      ws->X.ensure((size_t)mrows * ncols, st());
-      ma.mode = MPO_OUT_SVD_H;
-      ma.out = ws->X.p;
-      ma.ldx = mrows;
-      mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
-      const int k = ws->svd(mrows, ncols);
-      // new site [j][s][x] = V[(x*S + s) + perm[j]*ldv]   (a = x, b = s)
-      DTensor U = dalloc(k, S, Dx, 1, st());
-      ws->emit(ws->V.p, ncols, Dx, S, /*dj*/ (long)S * Dx, 0, /*da*/ 1, 0, /*db*/ Dx, 0,
-               0, U.p, k);
-      // next carry [l][j][w] = conj(Z[(l*Wl + w) + perm[j]*mrows])   (a = l, b = w)
-      ws->carry2.ensure((size_t)Dl * k * Wl, st());
-      ws->emit(ws->X.p, mrows, Dl, Wl, /*dj*/ Wl, 0, /*da*/ 0, Wl, /*db*/ 1, 0, 1,
-               ws->carry2.p, k);
+      int k;
+      DTensor U;
+      ws->carry2.ensure((size_t)Dl * k_bound(mrows, ncols) * Wl, st());
+      if (ncols <= kQrMaxRows) {
+        // M[(x,s),(l,w)] column-major (Dx*S) x (Dl*Wl) for the pivoted QR
+        ws->Q.ensure((size_t)mrows * ncols, st());
+        ma.mode = MPO_OUT_M;
+        ma.out = ws->Q.p;
+        ma.ldx = ncols;
+        mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        k = ws->svd_qrcp(ncols, mrows);
+        // new site [j][s][x]: row i = x*S + s (a = x, b = s)
+        U = dalloc(k, S, Dx, 1, st());
+        ws->emit_left_q(S, /*dj*/ (long)S * Dx, 0, /*da*/ 1, 0, /*db*/ Dx, 0, U.p, k);
+        // next carry [l][j][w]: column c = l*Wl + w of M (a = l, b = w)
+        ws->emit_svh(Wl, /*dj*/ Wl, 0, /*da*/ 0, Wl, /*db*/ 1, 0, ws->carry2.p, k);
+      } else {
+        // X = M^H with M[(x,s),(l,w)]: X is (Dl*Wl) x (Dx*S)
+        ws->X.ensure((size_t)mrows * ncols, st());
+        ma.mode = MPO_OUT_SVD_H;
+        ma.out = ws->X.p;
+        ma.ldx = mrows;
+        mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        k = ws->svd(mrows, ncols);
+        // new site [j][s][x] = V[(x*S + s) + perm[j]*ldv]   (a = x, b = s)
+        U = dalloc(k, S, Dx, 1, st());
+        ws->emit(ws->V.p, ncols, Dx, S, /*dj*/ (long)S * Dx, 0, /*da*/ 1, 0, /*db*/ Dx, 0,
+                 0, U.p, k);
+        // next carry [l][j][w] = conj(Z[(l*Wl + w) + perm[j]*mrows])   (a = l, b = w)
+        ws->emit(ws->X.p, mrows, Dl, Wl, /*dj*/ Wl, 0, /*da*/ 0, Wl, /*db*/ 1, 0, 1,
+                 ws->carry2.p, k);
+      }
       std::swap(ws->carry, ws->carry2);
       Cr = Dl; Dx = k; Cv = Wl;
       set_site(ia, U);
@@ -589,21 +717,38 @@ struct Chain {
         set_site(j, T);
         break;
       }
-      // X = M^H with M[(x,s),(r,e)]: X is (Dr*E) x (Dx*S)
       const int mrows = Dr * E, ncols = Dx * S;
-      ws->X.ensure((size_t)mrows * ncols, st());
-      ma.mode = MPO_OUT_SVD_H;
-      ma.out = ws->X.p;
-      ma.ldx = mrows;
-      mpo_apply_right_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
-      const int k = ws->svd(mrows, ncols);
-      // new site [x][s][j] = V[(x*S+s) + perm[j]*ldv]  (a = x*S+s)
-      DTensor U = dalloc(Dx, S, k, 1, st());
-      ws->emit(ws->V.p, ncols, Dx * S, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
-      // carry [(j,e)][r] = conj(Z[(r*E + e) + perm[j]*mrows])  (a = r, b = e)
-      ws->carry2.ensure((size_t)k * E * Dr, st());
-      ws->emit(ws->X.p, mrows, Dr, E, /*dj*/ (long)E * Dr, 0, /*da*/ 1, 0, /*db*/ Dr, 0, 1,
-               ws->carry2.p, k);
+      int k;
+      DTensor U;
+      ws->carry2.ensure((size_t)k_bound(mrows, ncols) * E * Dr, st());
+      if (ncols <= kQrMaxRows) {
+        // M[(x,s),(r,e)] column-major (Dx*S) x (Dr*E) for the pivoted QR
+        ws->Q.ensure((size_t)mrows * ncols, st());
+        ma.mode = MPO_OUT_M;
+        ma.out = ws->Q.p;
+        ma.ldx = ncols;
+        mpo_apply_right_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        k = ws->svd_qrcp(ncols, mrows);
+        // new site [x][s][j]: row i = x*S + s -> i*k + j
+        U = dalloc(Dx, S, k, 1, st());
+        ws->emit_left_q(1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, U.p, k);
+        // carry [(j,e)][r]: column c = r*E + e of M (a = r, b = e)
+        ws->emit_svh(E, /*dj*/ (long)E * Dr, 0, /*da*/ 1, 0, /*db*/ Dr, 0, ws->carry2.p, k);
+      } else {
+        // X = M^H with M[(x,s),(r,e)]: X is (Dr*E) x (Dx*S)
+        ws->X.ensure((size_t)mrows * ncols, st());
+        ma.mode = MPO_OUT_SVD_H;
+        ma.out = ws->X.p;
+        ma.ldx = mrows;
+        mpo_apply_right_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
+        k = ws->svd(mrows, ncols);
+        // new site [x][s][j] = V[(x*S+s) + perm[j]*ldv]  (a = x*S+s)
+        U = dalloc(Dx, S, k, 1, st());
+        ws->emit(ws->V.p, ncols, Dx * S, 1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, 0, U.p, k);
+        // carry [(j,e)][r] = conj(Z[(r*E + e) + perm[j]*mrows])  (a = r, b = e)
+        ws->emit(ws->X.p, mrows, Dr, E, /*dj*/ (long)E * Dr, 0, /*da*/ 1, 0, /*db*/ Dr, 0,
+                 1, ws->carry2.p, k);
+      }
       std::swap(ws->carry, ws->carry2);
       Cl = Dr; Cw = E; Dx = k;
       set_site(j, U);
@@ -1470,15 +1615,27 @@ int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap, dou
       const long cnt = (long)m * n;
       std::vector<cplx> h = to_cplx(M, cnt);
       DTensor dm = dupload(h, m, n, 1, 1, ws.stream);
-      ws.X.ensure(cnt, ws.stream);
-      // X = M^H (n x m, column-major) is the elementwise conjugate of M
-      conj_copy_kernel<<<grid_for(cnt), 256, 0, ws.stream>>>(dm.p, cnt, ws.X.p);
-      const int k = ws.svd(n, m);
+      int k;
       DevBuf u, svh;
-      u.ensure((size_t)m * k, ws.stream);
-      svh.ensure((size_t)k * n, ws.stream);
-      ws.emit(ws.V.p, m, m, 1, 1, 0, 0, 1, 0, 0, 0, u.p, k);
-      ws.emit(ws.X.p, n, n, 1, n, 0, 1, 0, 0, 0, 1, svh.p, k);
+      if (m <= kQrMaxRows) {
+        // column-major copy of M for the pivoted QR
+        ws.Q.ensure(cnt, ws.stream);
+        transpose_kernel<<<grid_for(cnt), 256, 0, ws.stream>>>(dm.p, m, n, ws.Q.p);
+        k = ws.svd_qrcp(m, n);
+        u.ensure((size_t)m * k, ws.stream);
+        svh.ensure((size_t)k * n, ws.stream);
+        ws.emit_left_q(1, 1, 0, 0, 1, 0, 0, u.p, k);
+        ws.emit_svh(1, n, 0, 1, 0, 0, 0, svh.p, k);
+      } else {
+        ws.X.ensure(cnt, ws.stream);
+        // X = M^H (n x m, column-major) is the elementwise conjugate of M
+        conj_copy_kernel<<<grid_for(cnt), 256, 0, ws.stream>>>(dm.p, cnt, ws.X.p);
+        k = ws.svd(n, m);
+        u.ensure((size_t)m * k, ws.stream);
+        svh.ensure((size_t)k * n, ws.stream);
+        ws.emit(ws.V.p, m, m, 1, 1, 0, 0, 1, 0, 0, 0, u.p, k);
+        ws.emit(ws.X.p, n, n, 1, n, 0, 1, 0, 0, 0, 1, svh.p, k);
+      }
       HIP_CHECK(hipMemcpyAsync(U, u.p, (size_t)m * k * sizeof(double2),
                                hipMemcpyDeviceToHost, ws.stream));
       HIP_CHECK(hipMemcpyAsync(SVh, svh.p, (size_t)k * n * sizeof(double2),

@@ -6,7 +6,7 @@
 namespace pte {
 
 enum NodeKind { NODE_DENSE = 0, NODE_DELTA = 1 };
-enum MpoOut { MPO_OUT_SVD_H = 0, MPO_OUT_SITE = 1 };
+enum MpoOut { MPO_OUT_SVD_H = 0, MPO_OUT_SITE = 1, MPO_OUT_M = 2 };
 
 struct GemmArgs {
   int M, N, K;
@@ -48,6 +48,7 @@ struct JacobiArgs {
   double2* gpart;     // partial Grams [pair][slice][32*32] when slices > 1
   double tol, abs_tol_scale;
   int max_sweeps;
+  int inner_sweeps;   // cyclic sweeps of the pair's 2JB x 2JB eigenproblem
   unsigned* bar;
   int* flags;
   double* fro2;
@@ -61,26 +62,45 @@ struct EmitArgs {
   int A, B;
   long dj0, djk, da0, dak, db0, dbk;
   int conj;
+  const int* rowmap;   // destination row of source row s (null: identity)
   double2* dst;
 };
 
-struct QrArgs {
+// Pivoted QR of a column-major matrix (see qrcp_kernel).
+struct QrcpCand {
+  double vmax;   // largest residual column norm of a workgroup
+  double vsum;   // sum of squared residual norms of a workgroup
+  int col;       // column of vmax (-1: none)
+  int pad;
+};
+
+struct QrcpArgs {
   double2* A; int m, n; long lda;
-  double2* tau;
+  double drop;                // stop when the residual norm <= drop * ||A||_F
+  double2* tau; double2* vscale; double* beta;
+  int* piv;                   // [n] physical column of each step, then the rest
+  int* done;                  // [n] scratch
+  QrcpCand* cand;             // [2 * gridDim.x] scratch
+  int* rank;                  // steps taken
   unsigned* bar;
 };
 
 struct ApplyQArgs {
-  const double2* A; int m, n; long lda;   // reflectors from qr_kernel
-  const double2* tau;
-  const double2* W; long ldw;             // n x n unitary of the Jacobi
+  const double2* A; int m; long lda;      // reflectors from qrcp_kernel
+  const double2* tau; const double2* vscale; const int* piv; const int* qrank;
+  const double2* W; long ldw;             // Jacobi unitary (qrank x qrank)
   const int* perm;
   const SvdInfo* info;
-  double2* dst; long dj, di;
+  int B;                                  // row i = a*B + b
+  long dj0, djk, da0, dak, db0, dbk;      // strides d0 + dk * rank
+  double2* dst;
 };
 
-__global__ void qr_kernel(QrArgs a);
-__global__ void build_rh_kernel(const double2* A, long lda, int n, double2* Y);
+constexpr int kQrcpMaxLocal = 64;         // columns per workgroup in qrcp_kernel
+
+__global__ void qrcp_kernel(QrcpArgs a);
+__global__ void build_rh_kernel(const double2* A, long lda, int n, const int* piv,
+                                const double* beta, const int* rank, double2* Y);
 __global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);

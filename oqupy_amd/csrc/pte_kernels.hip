@@ -18,6 +18,24 @@
 
 namespace pte {
 
+// Optional phase timers of the cooperative kernels (workgroup 0, thread 0,
+// s_memrealtime ticks), compiled in with -DPTE_PHASE_TIMING only.
+#ifdef PTE_PHASE_TIMING
+__device__ unsigned long long g_phase[16];
+#define PHASE_INIT() unsigned long long ph_t_ = wall_clock64()
+#define PHASE(k)                                                       \
+  do {                                                                 \
+    if (blockIdx.x == 0 && threadIdx.x == 0) {                         \
+      const unsigned long long t_ = wall_clock64();                    \
+      g_phase[k] += t_ - ph_t_;                                        \
+      ph_t_ = t_;                                                      \
+    }                                                                  \
+  } while (0)
+#else
+#define PHASE_INIT() (void)0
+#define PHASE(k) (void)0
+#endif
+
 // ---------------------------------------------------------------------------
 // complex GEMM:  C(i,j) = alpha * sum_k opA(A)(i,k) * opB(B)(k,j) [+ C(i,j)]
 // 64x64 output tile per 256-thread workgroup, 4x4 outputs per thread, K in
@@ -98,6 +116,7 @@ zgemm_kernel(GemmArgs a) {
 // T[x,s,l,w] = sum_{p,v} W[w,p,s,v] * G[(l,p),(x,v)]
 //   mode MPO_OUT_SVD_H : X[(l*Wl+w) + (x*S+s)*ldx] = conj(T)   (X = M^H)
 //   mode MPO_OUT_SITE  : out[(l*S+s)*Dx + x] = T   (requires Wl == 1)
+//   mode MPO_OUT_M     : out[(x*S+s) + (l*Wl+w)*ldx] = T     (M column-major)
 // node_kind DELTA: W[w,p,s,v] = delta(w,v) delta(p,s) I[w,p] (I is Wl x P).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
@@ -126,6 +145,8 @@ mpo_apply_kernel(MpoApplyArgs a) {
     }
     if (a.mode == MPO_OUT_SVD_H) {
       a.out[((long)l * a.Wl + w) + ((long)x * a.S + s) * a.ldx] = cconj(acc);
+    } else if (a.mode == MPO_OUT_M) {
+      a.out[((long)x * a.S + s) + ((long)l * a.Wl + w) * a.ldx] = acc;
     } else {
       a.out[((long)l * a.S + s) * a.Dx + x] = acc;
     }
@@ -136,6 +157,7 @@ mpo_apply_kernel(MpoApplyArgs a) {
 // TEMPO zip (left to right):
 // T[x,s,r,e] = sum_{p,w} W[w,p,s,e] * G[(x,w),(p,r)]
 //   mode MPO_OUT_SVD_H : X[(r*E+e) + (x*S+s)*ldx] = conj(T)
+//   mode MPO_OUT_M     : out[(x*S+s) + (r*E+e)*ldx] = T   (M column-major)
 //   mode MPO_OUT_SITE  : out[(x*S+s)*E + e] = T (requires Dr == 1)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
@@ -168,6 +190,8 @@ mpo_apply_right_kernel(MpoApplyArgs a) {
     }
     if (a.mode == MPO_OUT_SVD_H) {
       a.out[((long)r * E + e) + ((long)x * a.S + s) * a.ldx] = cconj(acc);
+    } else if (a.mode == MPO_OUT_M) {
+      a.out[((long)x * a.S + s) + ((long)r * E + e) * a.ldx] = acc;
     } else {
       a.out[((long)x * a.S + s) * E + e] = acc;
     }
@@ -182,7 +206,7 @@ mpo_apply_right_kernel(MpoApplyArgs a) {
 // round-robin tournament over nb blocks is handled by `slices` workgroups,
 // each owning a slice of the rows of X and of V. The pair's 2JB x 2JB Gram
 // matrix is summed from the slices' partial Grams, diagonalised (cyclic
-// two-sided Jacobi in LDS, at most INNER_SWEEPS sweeps, identically in every
+// two-sided Jacobi in LDS, at most inner_sweeps sweeps, identically in every
 // workgroup of the pair) and the resulting unitary applied to the slice's rows
 // of X and V. A sweep is nb-1 rounds; the iteration stops after a sweep in
 // which no pair needed a rotation. A pair needs one when some
@@ -192,7 +216,6 @@ constexpr int JB = 16;          // block width
 constexpr int JP = 2 * JB;      // columns per block pair
 constexpr int JR = 32;          // rows per streamed chunk
 constexpr int JT = 256;         // threads per workgroup
-constexpr int INNER_SWEEPS = 2;
 
 __device__ __forceinline__ void tournament_pair(int nplayers, int round, int slot,
                                                 int& a, int& b) {
@@ -210,33 +233,39 @@ __device__ __forceinline__ void ordered_pair(int sub, int slot, int& p, int& q) 
   if (p > q) { const int t = p; p = q; q = t; }
 }
 
-// Rotation of the Hermitian 2x2 block [[app, apq], [conj(apq), aqq]]:
-// J = [[cs, sn], [-sn conj(ph), cs conj(ph)]], ph = apq / |apq|, chosen so that
-// J^H [[app, apq], [conj(apq), aqq]] J is diagonal (Rutishauser's formulas).
+// Does the Hermitian 2x2 block [[app, apq], [conj(apq), aqq]] need a rotation:
+// |apq| > tol * sqrt(|app aqq|) and |apq| > abs_tol * sqrt(max(app, aqq)),
+// compared in squares.
+__device__ __forceinline__ bool needs_rotation(double app, double aqq, double mag2,
+                                               double tol2, double abs_tol2) {
+  return mag2 > tol2 * fabs(app * aqq) && mag2 > abs_tol2 * fmax(fmax(app, aqq), 0.0) &&
+         mag2 > 0.0;
+}
+
+// Rotation of that block: J = [[cs, sn], [-sn conj(ph), cs conj(ph)]],
+// ph = apq / |apq|, chosen so that J^H [[app, apq], [conj(apq), aqq]] J is
+// diagonal (Rutishauser's formulas). r = {cs, sn, Re ph, Im ph}.
 __device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 apq,
-                                                double tol, double abs_tol, double* r) {
-  const double mag = sqrt(cabs2(apq));
+                                                double tol2, double abs_tol2, double* r) {
+  const double mag2 = cabs2(apq);
   r[0] = 1.0; r[1] = 0.0; r[2] = 1.0; r[3] = 0.0;
-  if (!(mag > tol * sqrt(fabs(app * aqq)) &&
-        mag > abs_tol * sqrt(fmax(fmax(app, aqq), 0.0)) && mag > 0.0))
-    return false;
-  const double zeta = (aqq - app) / (2.0 * mag);
+  if (!needs_rotation(app, aqq, mag2, tol2, abs_tol2)) return false;
+  const double inv = rsqrt(mag2);
+  const double zeta = (aqq - app) * 0.5 * inv;
   const double t = (zeta == 0.0)
       ? 1.0 : copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double cs = 1.0 / sqrt(1.0 + t * t);
-  r[0] = cs; r[1] = cs * t; r[2] = apq.x / mag; r[3] = apq.y / mag;
+  const double cs = rsqrt(1.0 + t * t);
+  r[0] = cs; r[1] = cs * t; r[2] = apq.x * inv; r[3] = apq.y * inv;
   return true;
 }
 
 __global__ void __launch_bounds__(JT)
 jacobi_kernel(JacobiArgs a) {
   __shared__ double2 Xs[JR][JP + 1];
-  __shared__ double2 Gs[JP][JP + 1];
+  __shared__ double2 Gs[2][JP][JP + 1];
   __shared__ double2 Ws[JP][JP + 1];
-  __shared__ double rot[JB][4];
-  __shared__ int rotflag[JB];
   __shared__ int cols[JP];
-  __shared__ int need, anyrot;
+  __shared__ int need;
   __shared__ double red[JT];
 
   const int tid = threadIdx.x;
@@ -251,7 +280,7 @@ jacobi_kernel(JacobiArgs a) {
   for (long e = blockIdx.x * (long)JT + tid; e < (long)a.n * a.n;
        e += (long)nblk * JT) {
     const long i = e % a.n, j = e / a.n;
-    a.V[i + j * a.ldv] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    st_agent(a.V + (i + j * a.ldv), make_double2(i == j ? 1.0 : 0.0, 0.0));
   }
   double part = 0.0;
   for (long e = blockIdx.x * (long)JT + tid; e < (long)a.m * a.n;
@@ -270,13 +299,15 @@ jacobi_kernel(JacobiArgs a) {
   const double fro = sqrt(__hip_atomic_load(a.fro2, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT));
   const double abs_tol = a.abs_tol_scale * fro;
-  const double tol = a.tol;
+  const double tol2 = a.tol * a.tol, abs_tol2 = abs_tol * abs_tol;
   const int ti = tid >> 4, tj = tid & 15;        // 2x2 Gram blocking
 
   int sweep = 0;
   bool converged = false;
+  PHASE_INIT();
   for (; sweep < a.max_sweeps; ++sweep) {
     for (int round = 0; round < a.nb - 1; ++round) {
+      PHASE(7);
       int bi, bj;
       tournament_pair(a.nb, round, pair, bi, bj);
       if (tid < JP) {
@@ -296,7 +327,7 @@ jacobi_kernel(JacobiArgs a) {
           const int e = tid + JT * q;
           const int rr = e % JR, cc = e / JR;
           const int gr = r0 + rr, gc = cols[cc];
-          Xs[rr][cc] = (gr < x_hi && gc >= 0) ? a.X[gr + (long)gc * a.ldx]
+          Xs[rr][cc] = (gr < x_hi && gc >= 0) ? ld_agent(a.X + (gr + (long)gc * a.ldx))
                                               : make_double2(0.0, 0.0);
         }
         __syncthreads();
@@ -315,20 +346,22 @@ jacobi_kernel(JacobiArgs a) {
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
-          for (int v = 0; v < 2; ++v) Gs[ti + 16 * u][tj + 16 * v] = g[u][v];
+          for (int v = 0; v < 2; ++v) Gs[0][ti + 16 * u][tj + 16 * v] = g[u][v];
       } else {
         double2* mine = a.gpart + ((long)pair * R + slice) * (JP * JP);
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
-          for (int v = 0; v < 2; ++v) mine[(ti + 16 * u) * JP + tj + 16 * v] = g[u][v];
+          for (int v = 0; v < 2; ++v) st_agent(mine + ((ti + 16 * u) * JP + tj + 16 * v), g[u][v]);
+        PHASE(0);
         grid_barrier(a.bar, nblk, epoch);
+        PHASE(1);
         // identical summation order in every workgroup of the pair
         for (int e = tid; e < JP * JP; e += JT) {
           double2 acc = make_double2(0.0, 0.0);
           const double2* base = a.gpart + (long)pair * R * (JP * JP) + e;
-          for (int sl = 0; sl < R; ++sl) acc = cadd(acc, base[(long)sl * (JP * JP)]);
-          Gs[e / JP][e % JP] = acc;
+          for (int sl = 0; sl < R; ++sl) acc = cadd(acc, ld_agent(base + (long)sl * (JP * JP)));
+          Gs[0][e / JP][e % JP] = acc;
         }
       }
       for (int e = tid; e < JP * JP; e += JT)
@@ -341,88 +374,89 @@ jacobi_kernel(JacobiArgs a) {
         for (int e = tid; e < JP * JP; e += JT) {
           const int gi = e / JP, gj = e % JP;
           if (gi < gj && cols[gi] >= 0 && cols[gj] >= 0) {
-            const double dii = Gs[gi][gi].x, djj = Gs[gj][gj].x;
-            const double mag = sqrt(cabs2(Gs[gi][gj]));
-            if (mag > tol * sqrt(fabs(dii * djj)) &&
-                mag > abs_tol * sqrt(fmax(fmax(dii, djj), 0.0)))
+            if (needs_rotation(Gs[0][gi][gi].x, Gs[0][gj][gj].x, cabs2(Gs[0][gi][gj]), tol2,
+                               abs_tol2))
               mine = 1;
           }
         }
         if (mine) need = 1;
       }
       __syncthreads();
+      PHASE(2);
       if (need) {
         // ---- cyclic two-sided Jacobi on Gs, accumulating Ws ----
-        for (int isw = 0; isw < INNER_SWEEPS; ++isw) {
-          if (tid == 0) anyrot = 0;
+        // Gs is double-buffered: a sub-step reads Gs[cur] and writes every
+        // entry of Gs[cur ^ 1], so one barrier per sub-step suffices. Each
+        // thread owns the 2x2 block (P1, P2) of the sub-step's pairing and
+        // computes the two rotations it needs itself (identical in every
+        // thread); it also rotates rows P1 and P1+16 of Ws for pair P2.
+        const int P1 = tid >> 4, P2 = tid & 15;
+        int cur = 0;
+        for (int isw = 0; isw < a.inner_sweeps; ++isw) {
+          int any = 0;
           for (int sub = 0; sub < JP - 1; ++sub) {
-            if (tid < JB) {
-              int p, q;
-              ordered_pair(sub, tid, p, q);
-              double r[4];
-              const bool on = (cols[p] >= 0 && cols[q] >= 0) &&
-                  jacobi_rotation(Gs[p][p].x, Gs[q][q].x, Gs[p][q], tol, abs_tol, r);
-              rot[tid][0] = r[0]; rot[tid][1] = r[1]; rot[tid][2] = r[2]; rot[tid][3] = r[3];
-              rotflag[tid] = on;
-              if (on) anyrot = 1;
+            int p1, q1, p2, q2;
+            ordered_pair(sub, P1, p1, q1);
+            ordered_pair(sub, P2, p2, q2);
+            // every lane computes the rotation of pair P2 = lane & 15; the one
+            // of pair P1 comes from lane P1 of the same wavefront
+            double r1[4], r2[4];
+            const bool on2 = (cols[p2] >= 0 && cols[q2] >= 0) &&
+                jacobi_rotation(Gs[cur][p2][p2].x, Gs[cur][q2][q2].x, Gs[cur][p2][q2], tol2,
+                                abs_tol2, r2);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) r1[h] = __shfl(r2[h], P1, 64);
+            const bool on1 = __shfl((int)on2, P1, 64) != 0;
+            any |= (on1 || on2);
+            double2 b00 = Gs[cur][p1][p2], b01 = Gs[cur][p1][q2];
+            double2 b10 = Gs[cur][q1][p2], b11 = Gs[cur][q1][q2];
+            if (on2) {
+              const double c = r2[0], sn = r2[1];
+              const double2 cph = make_double2(r2[2], -r2[3]);
+              const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
+              const double2 n00 = make_double2(c * b00.x - sn * x01.x, c * b00.y - sn * x01.y);
+              const double2 n01 = make_double2(sn * b00.x + c * x01.x, sn * b00.y + c * x01.y);
+              const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
+              const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
+              b00 = n00; b01 = n01; b10 = n10; b11 = n11;
             }
-            __syncthreads();
-            {
-              // block (P1, P2): G[(p1,q1),(p2,q2)] <- J1^H G J2
-              const int P1 = tid >> 4, P2 = tid & 15;
-              if (rotflag[P1] || rotflag[P2]) {
-                int p1, q1, p2, q2;
-                ordered_pair(sub, P1, p1, q1);
-                ordered_pair(sub, P2, p2, q2);
-                double2 b00 = Gs[p1][p2], b01 = Gs[p1][q2];
-                double2 b10 = Gs[q1][p2], b11 = Gs[q1][q2];
-                if (rotflag[P2]) {
-                  const double c = rot[P2][0], sn = rot[P2][1];
-                  const double2 cph = make_double2(rot[P2][2], -rot[P2][3]);
-                  const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
-                  const double2 n00 = make_double2(c * b00.x - sn * x01.x, c * b00.y - sn * x01.y);
-                  const double2 n01 = make_double2(sn * b00.x + c * x01.x, sn * b00.y + c * x01.y);
-                  const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
-                  const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
-                  b00 = n00; b01 = n01; b10 = n10; b11 = n11;
-                }
-                if (rotflag[P1]) {
-                  const double c = rot[P1][0], sn = rot[P1][1];
-                  const double2 ph = make_double2(rot[P1][2], rot[P1][3]);
-                  const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
-                  const double2 n00 = make_double2(c * b00.x - sn * y10.x, c * b00.y - sn * y10.y);
-                  const double2 n01 = make_double2(c * b01.x - sn * y11.x, c * b01.y - sn * y11.y);
-                  const double2 n10 = make_double2(sn * b00.x + c * y10.x, sn * b00.y + c * y10.y);
-                  const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
-                  b00 = n00; b01 = n01; b10 = n10; b11 = n11;
-                }
-                if (P1 == P2) {
-                  b01 = make_double2(0.0, 0.0);
-                  b10 = make_double2(0.0, 0.0);
-                  b00.y = 0.0;
-                  b11.y = 0.0;
-                }
-                Gs[p1][p2] = b00; Gs[p1][q2] = b01; Gs[q1][p2] = b10; Gs[q1][q2] = b11;
-              }
-              // W[:, (p,q)] <- W[:, (p,q)] J for two rows per thread
-              const int pr = tid & 15;
-              if (rotflag[pr]) {
-                int p, q;
-                ordered_pair(sub, pr, p, q);
-                const double c = rot[pr][0], sn = rot[pr][1];
-                const double2 cph = make_double2(rot[pr][2], -rot[pr][3]);
-                for (int row = tid >> 4; row < JP; row += 16) {
-                  const double2 wp = Ws[row][p];
-                  const double2 wq = cmul(Ws[row][q], cph);
-                  Ws[row][p] = make_double2(c * wp.x - sn * wq.x, c * wp.y - sn * wq.y);
-                  Ws[row][q] = make_double2(sn * wp.x + c * wq.x, sn * wp.y + c * wq.y);
-                }
+            if (on1) {
+              const double c = r1[0], sn = r1[1];
+              const double2 ph = make_double2(r1[2], r1[3]);
+              const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
+              const double2 n00 = make_double2(c * b00.x - sn * y10.x, c * b00.y - sn * y10.y);
+              const double2 n01 = make_double2(c * b01.x - sn * y11.x, c * b01.y - sn * y11.y);
+              const double2 n10 = make_double2(sn * b00.x + c * y10.x, sn * b00.y + c * y10.y);
+              const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
+              b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+            }
+            if (P1 == P2 && on1) {
+              b01 = make_double2(0.0, 0.0);
+              b10 = make_double2(0.0, 0.0);
+              b00.y = 0.0;
+              b11.y = 0.0;
+            }
+            const int nxt = cur ^ 1;
+            Gs[nxt][p1][p2] = b00; Gs[nxt][p1][q2] = b01;
+            Gs[nxt][q1][p2] = b10; Gs[nxt][q1][q2] = b11;
+            if (on2) {
+              const double c = r2[0], sn = r2[1];
+              const double2 cph = make_double2(r2[2], -r2[3]);
+#pragma unroll
+              for (int h = 0; h < 2; ++h) {
+                const int row = P1 + 16 * h;
+                const double2 wp = Ws[row][p2];
+                const double2 wq = cmul(Ws[row][q2], cph);
+                Ws[row][p2] = make_double2(c * wp.x - sn * wq.x, c * wp.y - sn * wq.y);
+                Ws[row][q2] = make_double2(sn * wp.x + c * wq.x, sn * wp.y + c * wq.y);
               }
             }
             __syncthreads();
+            cur = nxt;
           }
-          if (!anyrot) break;
+          if (!__syncthreads_or(any)) break;
         }
+        PHASE(3);
         // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws over this slice ----
         const int rr = tid >> 3, c0 = (tid & 7) * 4;
         for (int mat = 0; mat < 2; ++mat) {
@@ -435,7 +469,7 @@ jacobi_kernel(JacobiArgs a) {
               const int e = tid + JT * q;
               const int r_ = e % JR, cc = e / JR;
               const int gr = r0 + r_, gc = cols[cc];
-              Xs[r_][cc] = (gr < hi && gc >= 0) ? base[gr + (long)gc * ld]
+              Xs[r_][cc] = (gr < hi && gc >= 0) ? ld_agent(base + (gr + (long)gc * ld))
                                                 : make_double2(0.0, 0.0);
             }
             __syncthreads();
@@ -454,14 +488,16 @@ jacobi_kernel(JacobiArgs a) {
 #pragma unroll
               for (int c = 0; c < 4; ++c) {
                 const int gc = cols[c0 + c];
-                if (gc >= 0) base[gr + (long)gc * ld] = o[c];
+                if (gc >= 0) st_agent(base + (gr + (long)gc * ld), o[c]);
               }
             }
           }
         }
-        if (tid == 0) a.flags[sweep] = 1;
+        if (tid == 0) st_agent(a.flags + sweep, 1);
+        PHASE(4);
       }
       grid_barrier(a.bar, nblk, epoch);
+      PHASE(5);
     }
     if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT) == 0) {
@@ -567,170 +603,284 @@ emit_kernel(EmitArgs a) {
   const long dj = a.dj0 + a.djk * (long)k;
   const long da = a.da0 + a.dak * (long)k;
   const long db = a.db0 + a.dbk * (long)k;
-  const long total = (long)a.A * a.B * k;
+  const int rows = a.A * a.B;
+  const long total = (long)rows * k;
   for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int b = e % a.B;
-    const long t = e / a.B;
-    const int aa = t % a.A;
-    const int j = t / a.A;
-    double2 v = a.src[((long)aa * a.B + b) + (long)a.perm[j] * a.lds];
+    const int s = e % rows;
+    const int j = e / rows;
+    const int c = a.rowmap ? a.rowmap[s] : s;
+    const int aa = c / a.B, b = c % a.B;
+    double2 v = a.src[s + (long)a.perm[j] * a.lds];
     if (a.conj) v.y = -v.y;
     a.dst[j * dj + aa * da + b * db] = v;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Householder QR of a tall matrix (preconditioner of the Jacobi SVD).
-// A (m x n, column-major, lda, m >= n) is overwritten by R (upper triangle)
-// and the reflectors v_j (below the diagonal, v_j[j] = 1 implicit), with
-// A = Q R, Q = H_0 H_1 ... H_{n-1}, H_j = I - tau_j v_j v_j^H (the LAPACK
-// zgeqr2/zlarfg convention). Column c belongs to workgroup c % gridDim.x;
-// at step j every workgroup applies H_j^H to its columns c > j, and the owner
-// of column j+1 then forms reflector j+1; one grid barrier per column.
+// Householder QR with column pivoting (preconditioner of the Jacobi SVD).
+// A (m x n, column-major, lda) -> A P = Q R with P the column permutation
+// piv (piv[t] = physical column of step t), Q = H_0 ... H_{r-1},
+// H_t = I - tau_t v_t v_t^H, v_t[t] = 1 and v_t[i] = A[i + piv[t]*lda] *
+// vscale[t] for i > t (the pivot column keeps its unscaled entries),
+// R[t][piv[t]] = beta[t] and R[t][c] = A[t + c*lda] for t above the diagonal.
+// The reflector convention is LAPACK's zlarfg, the column-norm downdate with
+// its cancellation guard is zlaqp2's. The factorisation stops at step r when
+// the Frobenius norm of the unfactored block is at most drop * ||A||_F; the
+// rows below r of R are dropped (a perturbation of that size).
+// Column c belongs to workgroup c % gridDim.x; every workgroup forms the
+// step's reflector from the pivot column itself, so one grid barrier per
+// step suffices: candidates (largest residual norm per workgroup) are posted
+// before it and reduced identically by every workgroup after it.
 // ---------------------------------------------------------------------------
-__device__ double wg_sum(double v, double* red) {
-  const int tid = threadIdx.x;
-  __syncthreads();
-  red[tid] = v;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (tid < s) red[tid] += red[tid + s];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
+// (value, column) maximum with ties to the smaller column; -1 = none
+__device__ __forceinline__ void better(double& v, int& c, double ov, int oc) {
+  if (oc >= 0 && (c < 0 || ov > v || (ov == v && oc < c))) { v = ov; c = oc; }
 }
 
-__device__ void make_reflector(double2* A, int m, long lda, int j, double2* tau,
-                               double* red) {
-  double2* col = A + (long)j * lda;
-  double part = 0.0;
-  for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) part += cabs2(col[i]);
-  const double xnorm2 = wg_sum(part, red);
-  const double2 alpha = col[j];
-  if (xnorm2 == 0.0 && alpha.y == 0.0) {
-    if (threadIdx.x == 0) tau[j] = make_double2(0.0, 0.0);
-    return;
+// global winner and residual total among the posted candidates; every
+// thread of the workgroup gets identical results
+__device__ void qrcp_reduce(const QrcpCand* cand, int nblk, double (*slot)[4],
+                            double (*slotv)[4], int (*slotc)[4], int& parity,
+                            int* bestc, double* total) {
+  double bv = -1.0, sv = 0.0;
+  int bc = -1;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    sv += ld_agent(&cand[b].vsum);
+    better(bv, bc, ld_agent(&cand[b].vmax), ld_agent(&cand[b].col));
   }
-  const double beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2),
-                                alpha.x);
-  // scale = 1 / (alpha - beta)
-  const double dr = alpha.x - beta, di = alpha.y;
-  const double den = dr * dr + di * di;
-  const double2 scale = make_double2(dr / den, -di / den);
-  for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) col[i] = cmul(col[i], scale);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(bv, off, 64);
+    const int oc = __shfl_xor(bc, off, 64);
+    better(bv, bc, ov, oc);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    slotv[parity][threadIdx.x >> 6] = bv;
+    slotc[parity][threadIdx.x >> 6] = bc;
+  }
+  // block_sum256 synchronises before reading, which also publishes the slots
+  *total = block_sum256(sv, slot, parity);
+  double v = -1.0;
+  int c = -1;
+  const int par = parity ^ 1;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) better(v, c, slotv[par][w], slotc[par][w]);
+  *bestc = c;
+}
+
+// wavefront-level column work of qrcp_kernel: the four wavefronts of a
+// workgroup take its columns in turn
+__device__ __forceinline__ void qrcp_post(QrcpCand* dst, double wmax, int wcol, double wsum,
+                                          double* cw_max, int* cw_col, double* cw_sum) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) { cw_max[wave] = wmax; cw_col[wave] = wcol; cw_sum[wave] = wsum; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    tau[j] = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-    col[j] = make_double2(beta, 0.0);
-  }
-}
-
-__global__ void __launch_bounds__(256)
-qr_kernel(QrArgs a) {
-  __shared__ double red[256];
-  __shared__ double redi[256];
-  const unsigned nblk = gridDim.x;
-  unsigned epoch = 0;
-  if (a.n > 0 && 0 % (int)nblk == (int)blockIdx.x) make_reflector(a.A, a.m, a.lda, 0, a.tau, red);
-  grid_barrier(a.bar, nblk, epoch);
-  for (int j = 0; j + 1 < a.n; ++j) {
-    const double2* v = a.A + (long)j * a.lda;
-    const double2 t = a.tau[j];
-    const double2 tc = make_double2(t.x, -t.y);   // apply H_j^H
-    int c0 = j + 1 + (((int)blockIdx.x - (j + 1)) % (int)nblk + (int)nblk) % (int)nblk;
-    for (int c = c0; c < a.n; c += nblk) {
-      double2* col = a.A + (long)c * a.lda;
-      if (t.x != 0.0 || t.y != 0.0) {
-        double pr = 0.0, pi = 0.0;
-        for (int i = j + threadIdx.x; i < a.m; i += blockDim.x) {
-          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-          const double2 x = col[i];
-          pr += vi.x * x.x + vi.y * x.y;      // conj(v) * x
-          pi += vi.x * x.y - vi.y * x.x;
-        }
-        __syncthreads();
-        red[threadIdx.x] = pr;
-        redi[threadIdx.x] = pi;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-          if (threadIdx.x < s) {
-            red[threadIdx.x] += red[threadIdx.x + s];
-            redi[threadIdx.x] += redi[threadIdx.x + s];
-          }
-          __syncthreads();
-        }
-        const double2 w = make_double2(red[0], redi[0]);
-        const double2 tw = cmul(tc, w);
-        for (int i = j + threadIdx.x; i < a.m; i += blockDim.x) {
-          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : v[i];
-          col[i] = csub(col[i], cmul(vi, tw));
-        }
-        __syncthreads();
-      }
-      if (c == j + 1) make_reflector(a.A, a.m, a.lda, j + 1, a.tau, red);
+    double v = -1.0, sm = 0.0;
+    int c = -1;
+    for (int w = 0; w < 4; ++w) {
+      sm += cw_sum[w];
+      better(v, c, cw_max[w], cw_col[w]);
     }
-    grid_barrier(a.bar, nblk, epoch);
+    st_agent(&dst->vmax, v);
+    st_agent(&dst->vsum, sm);
+    st_agent(&dst->col, c);
   }
 }
 
-// Y = R^H (n x n, column-major) from the upper triangle of A (lda)
 __global__ void __launch_bounds__(256)
-build_rh_kernel(const double2* A, long lda, int n, double2* Y) {
-  const long total = (long)n * n;
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
-    const int i = e % n;
-    const int c = e / n;
-    Y[e] = (c <= i) ? cconj(A[c + (long)i * lda]) : make_double2(0.0, 0.0);
+qrcp_kernel(QrcpArgs a) {
+  extern __shared__ double2 vcol[];            // reflector of the step (m)
+  __shared__ double slot[2][4], slotv[2][4];
+  __shared__ int slotc[2][4];
+  __shared__ double vn1[kQrcpMaxLocal], vn2[kQrcpMaxLocal];
+  __shared__ int done[kQrcpMaxLocal];
+  __shared__ double cw_max[4], cw_sum[4];
+  __shared__ int cw_col[4];
+  int parity = 0;
+  const int nblk = gridDim.x, blk = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int m = a.m, n = a.n;
+  const long lda = a.lda;
+  const int kmax = m < n ? m : n;
+  const int nloc = (n - blk + nblk - 1) / nblk;   // own columns blk, blk+nblk, ...
+  unsigned epoch = 0;
+
+  // initial column norms (one wavefront per column)
+  double wmax = -1.0, wsum = 0.0;
+  int wcol = -1;
+  for (int q = wave; q < nloc; q += 4) {
+    const int c = blk + q * nblk;
+    const double2* col = a.A + (long)c * lda;
+    double part = 0.0;
+    for (int i = lane; i < m; i += 64) part += cabs2(col[i]);
+    const double nrm = sqrt(wave_sum(part));
+    if (lane == 0) { vn1[q] = nrm; vn2[q] = nrm; done[q] = 0; st_agent(a.done + c, 0); }
+    wsum += nrm * nrm;
+    better(wmax, wcol, nrm, c);
+  }
+  qrcp_post(a.cand + blk, wmax, wcol, wsum, cw_max, cw_col, cw_sum);
+  grid_barrier(a.bar, nblk, epoch);
+  double total;
+  int p;
+  qrcp_reduce(a.cand, nblk, slot, slotv, slotc, parity, &p, &total);
+  const double drop2 = a.drop * a.drop * total;
+  const double tol3z = 1.4901161193847656e-08;   // sqrt(eps), zlaqp2
+  int j = 0;
+  PHASE_INIT();
+  while (j < kmax && p >= 0 && total > drop2) {
+    // reflector j from the pivot column p (rows j..m-1), whole workgroup
+    const double2* pc = a.A + (long)p * lda;
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += 256) part += cabs2(ld_agent(pc + i));
+    const double xnorm2 = block_sum256(part, slot, parity);
+    const double2 alpha = ld_agent(pc + j);
+    double2 tau, scale;
+    double beta;
+    if (xnorm2 == 0.0 && alpha.y == 0.0) {
+      tau = make_double2(0.0, 0.0);
+      scale = make_double2(0.0, 0.0);
+      beta = alpha.x;
+    } else {
+      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
+      const double dr = alpha.x - beta, di = alpha.y;
+      const double den = dr * dr + di * di;
+      scale = make_double2(dr / den, -di / den);
+      tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    }
+    for (int i = j + tid; i < m; i += 256)
+      vcol[i] = (i == j) ? make_double2(1.0, 0.0) : cmul(ld_agent(pc + i), scale);
+    if (blk == 0 && tid == 0) {
+      a.piv[j] = p;
+      a.tau[j] = tau;
+      a.vscale[j] = scale;
+      a.beta[j] = beta;
+    }
+    if (p % nblk == blk && tid == 0) {
+      done[p / nblk] = 1;
+      st_agent(a.done + p, 1);
+    }
+    __syncthreads();
+    PHASE(8);
+    // apply H_j^H to the own remaining columns, downdate their norms
+    const double2 tc = make_double2(tau.x, -tau.y);
+    const bool reflect = (tau.x != 0.0 || tau.y != 0.0);
+    wmax = -1.0; wsum = 0.0; wcol = -1;
+    for (int q = wave; q < nloc; q += 4) {
+      if (done[q]) continue;
+      const int c = blk + q * nblk;
+      double2* col = a.A + (long)c * lda;
+      if (reflect) {
+        double2 w = make_double2(0.0, 0.0);
+        for (int i = j + lane; i < m; i += 64) cfmac(w, vcol[i], ld_agent(col + i));
+        w.x = wave_sum(w.x);
+        w.y = wave_sum(w.y);
+        const double2 tw = cmul(tc, w);
+        for (int i = j + lane; i < m; i += 64)
+          st_agent(col + i, csub(ld_agent(col + i), cmul(vcol[i], tw)));
+      }
+      double2 rj2 = (lane == 0) ? ld_agent(col + j) : make_double2(0.0, 0.0);
+      rj2.x = __shfl(rj2.x, 0, 64);
+      rj2.y = __shfl(rj2.y, 0, 64);
+      double nv = vn1[q];
+      if (nv != 0.0) {
+        const double rj = sqrt(cabs2(rj2));
+        double temp = 1.0 - (rj / nv) * (rj / nv);
+        temp = temp > 0.0 ? temp : 0.0;
+        const double ratio = nv / vn2[q];
+        const double temp2 = temp * ratio * ratio;
+        if (temp2 <= tol3z) {
+          // the column's new entries were stored by other lanes of this
+          // wavefront: complete the stores before reading them back
+          __threadfence_block();
+          double pp = 0.0;
+          for (int i = j + 1 + lane; i < m; i += 64) pp += cabs2(ld_agent(col + i));
+          nv = (j + 1 < m) ? sqrt(wave_sum(pp)) : 0.0;
+          if (lane == 0) { vn1[q] = nv; vn2[q] = nv; }
+        } else {
+          nv = nv * sqrt(temp);
+          if (lane == 0) vn1[q] = nv;
+        }
+      }
+      wsum += nv * nv;
+      better(wmax, wcol, nv, c);
+    }
+    PHASE(9);
+    qrcp_post(a.cand + ((j + 1) & 1) * nblk + blk, wmax, wcol, wsum, cw_max, cw_col,
+              cw_sum);
+    PHASE(10);
+    grid_barrier(a.bar, nblk, epoch);
+    PHASE(11);
+    qrcp_reduce(a.cand + ((j + 1) & 1) * nblk, nblk, slot, slotv, slotc, parity, &p,
+                &total);
+    PHASE(12);
+    ++j;
+  }
+  // remaining columns in ascending order after the pivots
+  if (blk == 0 && tid == 0) {
+    int t = j;
+    for (int c = 0; c < n; ++c)
+      if (!ld_agent(a.done + c)) a.piv[t++] = c;
+    *a.rank = j;
   }
 }
 
-// Left singular vectors Q W[:, perm[j]] (j < k): one workgroup per column,
-// the column held in LDS, reflectors applied from the last to the first;
-// written to dst[j*dj + i*di] for i < m.
+// Y = R^H (n x r, column-major, ld = n) in the pivoted column order:
+// Y[s + t*n] = conj(R[t][piv[s]])
+__global__ void __launch_bounds__(256)
+build_rh_kernel(const double2* A, long lda, int n, const int* piv, const double* beta,
+                const int* rank, double2* Y) {
+  const int r = *rank;
+  const long total = (long)n * r;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int s = e % n;
+    const int t = e / n;
+    double2 v = make_double2(0.0, 0.0);
+    if (t < s) v = cconj(A[t + (long)piv[s] * lda]);
+    else if (t == s) v = make_double2(beta[s], 0.0);
+    Y[e] = v;
+  }
+}
+
+// Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR: one
+// workgroup per column, the column held in LDS, reflectors applied from
+// the last to the first; row i = a*B + b goes to dst[j*dj + a*da + b*db].
 __global__ void __launch_bounds__(256)
 apply_q_kernel(ApplyQArgs a) {
   extern __shared__ double2 cvec[];
-  __shared__ double red[256];
-  __shared__ double redi[256];
+  __shared__ double slot[2][4], sloti[2][4];
+  int parity = 0;
   const int k = a.info->rank;
+  const int r = *a.qrank;
+  const long dj = a.dj0 + a.djk * (long)k;
+  const long da = a.da0 + a.dak * (long)k;
+  const long db = a.db0 + a.dbk * (long)k;
   for (int j = blockIdx.x; j < k; j += gridDim.x) {
     const double2* w = a.W + (long)a.perm[j] * a.ldw;
     for (int i = threadIdx.x; i < a.m; i += blockDim.x)
-      cvec[i] = (i < a.n) ? w[i] : make_double2(0.0, 0.0);
+      cvec[i] = (i < r) ? w[i] : make_double2(0.0, 0.0);
     __syncthreads();
-    for (int r = a.n - 1; r >= 0; --r) {
-      const double2 t = a.tau[r];
-      if (t.x == 0.0 && t.y == 0.0) continue;
-      const double2* v = a.A + (long)r * a.lda;
-      double pr = 0.0, pi = 0.0;
-      for (int i = r + threadIdx.x; i < a.m; i += blockDim.x) {
-        const double2 vi = (i == r) ? make_double2(1.0, 0.0) : v[i];
-        const double2 x = cvec[i];
-        pr += vi.x * x.x + vi.y * x.y;
-        pi += vi.x * x.y - vi.y * x.x;
+    for (int t = r - 1; t >= 0; --t) {
+      const double2 tau = a.tau[t];
+      if (tau.x == 0.0 && tau.y == 0.0) continue;
+      const double2* v = a.A + (long)a.piv[t] * a.lda;
+      const double2 sc = a.vscale[t];
+      double2 acc = make_double2(0.0, 0.0);
+      for (int i = t + threadIdx.x; i < a.m; i += blockDim.x) {
+        const double2 vi = (i == t) ? make_double2(1.0, 0.0) : cmul(v[i], sc);
+        cfmac(acc, vi, cvec[i]);
       }
-      red[threadIdx.x] = pr;
-      redi[threadIdx.x] = pi;
-      __syncthreads();
-      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-          red[threadIdx.x] += red[threadIdx.x + s];
-          redi[threadIdx.x] += redi[threadIdx.x + s];
-        }
-        __syncthreads();
-      }
-      const double2 tw = cmul(t, make_double2(red[0], redi[0]));
-      __syncthreads();
-      for (int i = r + threadIdx.x; i < a.m; i += blockDim.x) {
-        const double2 vi = (i == r) ? make_double2(1.0, 0.0) : v[i];
+      const double2 tw = cmul(tau, block_sum256c(acc, slot, sloti, parity));
+      for (int i = t + threadIdx.x; i < a.m; i += blockDim.x) {
+        const double2 vi = (i == t) ? make_double2(1.0, 0.0) : cmul(v[i], sc);
         cvec[i] = csub(cvec[i], cmul(vi, tw));
       }
       __syncthreads();
     }
-    for (int i = threadIdx.x; i < a.m; i += blockDim.x)
-      a.dst[(long)j * a.dj + (long)i * a.di] = cvec[i];
+    for (int i = threadIdx.x; i < a.m; i += blockDim.x) {
+      const int aa = i / a.B, b = i % a.B;
+      a.dst[(long)j * dj + (long)aa * da + (long)b * db] = cvec[i];
+    }
     __syncthreads();
   }
 }
@@ -795,3 +945,16 @@ identity_carry_kernel(double2* dst, int R, int V) {
 }
 
 }  // namespace pte
+
+#ifdef PTE_PHASE_TIMING
+// phase timer readout (s_memrealtime ticks); resets the timers
+extern "C" int pte_debug_phases(unsigned long long* out) {
+  if (hipDeviceSynchronize() != hipSuccess) return -3;
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(pte::g_phase), sizeof(unsigned long long) * 16) !=
+      hipSuccess)
+    return -3;
+  unsigned long long z[16] = {0};
+  if (hipMemcpyToSymbol(HIP_SYMBOL(pte::g_phase), z, sizeof(z)) != hipSuccess) return -3;
+  return 0;
+}
+#endif
