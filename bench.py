@@ -79,6 +79,19 @@ def gpu_run(warmup, steps, table):
     return elapsed, svd, window, backend.step
 
 
+def blas_threads():
+    """Threads of the BLAS/LAPACK library numpy uses (the oracle's cores)."""
+    try:
+        from threadpoolctl import threadpool_info
+        n = [d.get("num_threads", 0) for d in threadpool_info()
+             if d.get("user_api") == "blas"]
+        if n:
+            return int(max(n))
+    except Exception:
+        pass
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+
+
 def cpu_baseline(warmup, steps, table, budget_s):
     """The oracle on the same window (capped at ``budget_s`` seconds of timed
     work): steps/s over the steps it completed."""
@@ -134,7 +147,7 @@ def main():
     }
     cpu = None
     if not args.no_cpu:
-        cores = os.cpu_count()
+        cores = blas_threads()
         cpu_value, done = cpu_baseline(args.warmup, args.steps, table,
                                        args.cpu_budget)
         cpu = {"value": round(cpu_value, 6), "unit": "time-steps/s",
