@@ -52,12 +52,13 @@ def influence_table(bath, params):
             for dk in range(DKMAX + 1)}
 
 
-def gpu_run(warmup, steps, table):
+def gpu_run(warmup, steps, table, precision="c128"):
     import torch
     import oqupy_amd as oqupy
     bath, params, np = build_model()
     ptt = oqupy.PtTempo(bath, 0.0, NSTEPS * DT + 1e-9, params,
                         backend_config={"backend": "mi355x",
+                                        "precision": precision,
                                         "chi_cap": 8192})
     assert ptt.num_steps == NSTEPS
     ptt._influence = lambda dk: table.get(dk)
@@ -120,6 +121,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of timed oracle work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("c128", "c64"), default="c128",
+                    help="c64: contractions in complex64 on the matrix cores "
+                         "(FP32 accumulation), SVD and cut in FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; a "
@@ -132,7 +136,8 @@ def main():
     table = influence_table(bath, params)
     precompute_s = time.perf_counter() - t0
 
-    elapsed, svd, window, last_step = gpu_run(args.warmup, args.steps, table)
+    elapsed, svd, window, last_step = gpu_run(args.warmup, args.steps, table,
+                                              args.precision)
     value = args.steps / elapsed
     svd_avg_ms = svd["ms"] / max(svd["count"], 1)
     achieved = svd["flops"] / max(svd["ms"], 1e-12) / 1e9   # TFLOP/s
@@ -161,13 +166,15 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128", "data": "synthetic",
+        "dtype": args.precision, "data": "synthetic",
         "config": {
             "workload": (f"spin-boson PT-TEMPO alpha={ALPHA} wc={CUTOFF} "
                          f"T={TEMP} dt={DT} dkmax={DKMAX} epsrel={EPSREL} "
                          f"N={NSTEPS}; timed steps {args.warmup + 2}.."
                          f"{args.warmup + 1 + args.steps}"),
-            "precision": "plain complex128",
+            "precision": ("plain complex128" if args.precision == "c128" else
+                          "complex64 contractions (FP32 accumulation), "
+                          "FP64 SVD and cut"),
             "max_bond_in_window": max(w[2] for w in window),
             "svds_per_step": round(sum(w[0] for w in window) / len(window), 1),
             "jacobi_sweeps_per_svd": round(

@@ -29,7 +29,10 @@ extern "C" {
 #define PTE_ERR_CAPACITY (-4)  /* a bond would exceed the configured capacity */
 #define PTE_ERR_NUMERIC (-5)   /* non-finite input or SVD not converged      */
 
-#define PTE_PRECISION_C128 0
+#define PTE_PRECISION_C128 0  /* complex128 everywhere                          */
+#define PTE_PRECISION_C64 1   /* network contractions in complex64 on the matrix
+                                 cores with FP32 accumulation; QR, Jacobi SVD,
+                                 singular values and the epsrel cut in FP64   */
 
 typedef struct pte_pt_engine pte_pt_engine;
 typedef struct pte_tempo_engine pte_tempo_engine;
@@ -41,7 +44,7 @@ typedef struct {
   int num_steps;    /* N, number of time steps of the process tensor (>= 2)   */
   int dkmax;        /* memory cut K (the backend's dkmax, never None)         */
   double epsrel;    /* relative SVD truncation error                          */
-  int precision;    /* PTE_PRECISION_C128                                     */
+  int precision;    /* PTE_PRECISION_C128 or PTE_PRECISION_C64                */
   int chi_cap;      /* capacity of every bond; exceeding it is an error       */
   int n_north;      /* length of sum_north (d^2, or the degeneracy classes)   */
 } pte_pt_config;
@@ -187,6 +190,15 @@ int pte_dynamics_run_engine(pte_pt_engine* e, const double* props,
 int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap,
                   double* U, double* S, double* SVh, int* k_out,
                   int* sweeps_out);
+
+/* One network contraction C = A B (A: m x k, B: k x n, C: m x n, row-major
+ * complex128) by the kernels of the given precision (PTE_PRECISION_C64
+ * rounds the operands to complex64 and accumulates in FP32 on the matrix
+ * cores). Replaces one pairwise contraction of tensornetwork's greedy
+ * contractor as called at oqupy/backends/node_array.py:519 (zip-up) and the
+ * `@` absorption at node_array.py:272, 295. */
+int pte_contract(int precision, int m, int n, int k, const double* A, const double* B,
+                 double* C);
 
 const char* pte_last_error(void);
 

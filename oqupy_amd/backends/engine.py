@@ -98,7 +98,11 @@ _SIGNATURES = {
     "pte_split_svd": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _DP,
                                      ctypes.c_double, ctypes.c_int, _DP, _DP,
                                      _DP, _IP, _IP]),
+    "pte_contract": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _DP, _DP, _DP]),
 }
+
+PRECISIONS = {"c128": 0, "c64": 1}
 
 
 def exported_symbols():
@@ -178,12 +182,29 @@ def parse_backend_config(config: Optional[dict]) -> dict:
         raise ValueError(f"Unknown backend '{backend}'; this package provides "
                          "only the 'mi355x' engine.")
     precision = cfg.get("precision", "c128")
-    if precision != "c128":
-        raise ValueError(f"Unsupported precision '{precision}'.")
+    if precision not in PRECISIONS:
+        raise ValueError(f"Unsupported precision '{precision}'; use 'c128' "
+                         "or 'c64'.")
     chi_cap = int(cfg.get("chi_cap", DEFAULT_CHI_CAP))
     if chi_cap < 1:
         raise ValueError("chi_cap must be positive.")
     return {"backend": backend, "precision": precision, "chi_cap": chi_cap}
+
+
+def contract(a: np.ndarray, b: np.ndarray, precision: str = "c128"):
+    """``a @ b`` by the engine's contraction kernels of the given precision."""
+    require_gpu()
+    lib = load_library()
+    a = as_c128(a)
+    b = as_c128(b)
+    m, k = a.shape
+    k2, n = b.shape
+    if k != k2:
+        raise ValueError("inner dimensions differ")
+    c = np.empty((m, n), dtype=np.complex128)
+    check(lib.pte_contract(PRECISIONS[precision], m, n, k, dptr(a), dptr(b),
+                           dptr(c)))
+    return c
 
 
 def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192):

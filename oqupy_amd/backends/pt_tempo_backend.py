@@ -61,7 +61,8 @@ class PtTempoBackend:
         eng.require_gpu()
         cfg = eng._PtConfig(dimension=self._dimension,
                             num_steps=self._num_steps, dkmax=self._dkmax,
-                            epsrel=self._epsrel, precision=0,
+                            epsrel=self._epsrel,
+                            precision=eng.PRECISIONS[self._config["precision"]],
                             chi_cap=self._config["chi_cap"],
                             n_north=int(self._sum_north.size))
         handle = ctypes.c_void_p()

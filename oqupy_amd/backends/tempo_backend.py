@@ -62,7 +62,8 @@ class TempoBackend:
         cfg = eng._TempoConfig(
             dimension=self._dim,
             dkmax=-1 if self._dkmax is None else self._dkmax,
-            epsrel=self._epsrel, precision=0,
+            epsrel=self._epsrel,
+                            precision=eng.PRECISIONS[self._config["precision"]],
             chi_cap=self._config["chi_cap"],
             n_north=int(self._sum_north.size),
             n_west=int(self._sum_west.size))

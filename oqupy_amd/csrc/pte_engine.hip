@@ -201,6 +201,7 @@ struct Workspace {
   int max_coop_blocks = 0;
   double epsrel = 1e-7;
   int chi_cap = 512;
+  int precision = PTE_PRECISION_C128;
   // statistics of the current step
   int n_svd = 0, sweeps = 0, max_bond = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -475,6 +476,26 @@ struct Workspace {
     dim3 grid((N + 63) / 64, (M + 63) / 64);
     zgemm_kernel<<<grid, 256, 0, stream>>>(g);
   }
+
+  // A network contraction of the zip-up or a sweep: complex64 on the matrix
+  // cores in the precision form, complex128 otherwise.
+  void contract(int M, int N, int K, const double2* A, long sai, long sak,
+                const double2* B, long sbk, long sbj, double2* C, long sci, long scj) {
+    if (precision != PTE_PRECISION_C64) {
+      gemm(M, N, K, A, sai, sak, B, sbk, sbj, C, sci, scj);
+      return;
+    }
+    GemmArgs g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.sai = sai; g.sak = sak;
+    g.B = B; g.sbk = sbk; g.sbj = sbj;
+    g.C = C; g.sci = sci; g.scj = scj;
+    g.conjA = 0; g.conjB = 0;
+    g.alpha = make_double2(1.0, 0.0);
+    g.beta_zero = 1;
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    cgemm_mfma_kernel<<<grid, 256, 0, stream>>>(g);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -529,7 +550,7 @@ struct Chain {
       require(Bn.d[0] == Dr, PTE_ERR_ARG, "internal: bond mismatch in sweep");
       const int Pq = Bn.d[1], Ds = Bn.d[2];
       DTensor Nn = dalloc(k, Pq, Ds, 1, st());
-      ws->gemm(k, Pq * Ds, Dr, ws->svh.p, Dr, 1, Bn.p, (long)Pq * Ds, 1, Nn.p,
+      ws->contract(k, Pq * Ds, Dr, ws->svh.p, Dr, 1, Bn.p, (long)Pq * Ds, 1, Nn.p,
                (long)Pq * Ds, 1);
       set_site(i, U);
       set_site(i + 1, Nn);
@@ -574,7 +595,7 @@ struct Chain {
       require(Bp.d[2] == Dl, PTE_ERR_ARG, "internal: bond mismatch in sweep");
       const int Dl2 = Bp.d[0], Pq = Bp.d[1];
       DTensor Np = dalloc(Dl2, Pq, k, 1, st());
-      ws->gemm(Dl2 * Pq, k, Dl, Bp.p, Dl, 1, ws->svh.p, 1, Dl, Np.p, k, 1);
+      ws->contract(Dl2 * Pq, k, Dl, Bp.p, Dl, 1, ws->svh.p, 1, Dl, Np.p, k, 1);
       set_site(i, U);
       set_site(i - 1, Np);
     }
@@ -607,7 +628,7 @@ struct Chain {
       const int Wl = W.wl, S = W.s;
       // G[(l,p),(x,v)] = sum_r A[(l,p), r] carry[r, (x,v)]
       ws->G.ensure((size_t)Dl * P * Dx * Cv, st());
-      ws->gemm(Dl * P, Dx * Cv, Dr, A.p, Dr, 1, ws->carry.p, (long)Dx * Cv, 1, ws->G.p,
+      ws->contract(Dl * P, Dx * Cv, Dr, A.p, Dr, 1, ws->carry.p, (long)Dx * Cv, 1, ws->G.p,
                (long)Dx * Cv, 1);
       MpoApplyArgs ma;
       ma.G = ws->G.p;
@@ -698,7 +719,7 @@ struct Chain {
       const int E = W.wr, S = W.s;
       // G[(x,w),(p,r)] = sum_l carry[(x,w), l] A[l, (p,r)]
       ws->G.ensure((size_t)Dx * Cw * P * Dr, st());
-      ws->gemm(Dx * Cw, P * Dr, Dl, ws->carry.p, Dl, 1, A.p, (long)P * Dr, 1, ws->G.p,
+      ws->contract(Dx * Cw, P * Dr, Dl, ws->carry.p, Dl, 1, A.p, (long)P * Dr, 1, ws->G.p,
                (long)P * Dr, 1);
       MpoApplyArgs ma;
       ma.G = ws->G.p;
@@ -793,6 +814,7 @@ struct PtEngine {
 
   void init_engine() {
     ws.init(cfg.epsrel, cfg.chi_cap);
+    ws.precision = cfg.precision;
     mps.ws = &ws;
     num_infl = std::min(cfg.num_steps, cfg.dkmax + 1);
     infl.resize(num_infl);
@@ -1008,6 +1030,7 @@ struct TempoEngine {
 
   void init_engine() {
     ws.init(cfg.epsrel, cfg.chi_cap);
+    ws.precision = cfg.precision;
     mps.ws = &ws;
   }
 
@@ -1386,8 +1409,8 @@ int pte_pt_create(const pte_pt_config* cfg, pte_pt_engine** out) {
     require(cfg->dkmax >= 1, PTE_ERR_ARG, "dkmax must be >= 1");
     require(cfg->epsrel > 0.0 && std::isfinite(cfg->epsrel), PTE_ERR_ARG,
             "epsrel must be positive");
-    require(cfg->precision == PTE_PRECISION_C128, PTE_ERR_ARG,
-            "only the complex128 precision is available");
+    require(cfg->precision == PTE_PRECISION_C128 || cfg->precision == PTE_PRECISION_C64,
+            PTE_ERR_ARG, "precision must be PTE_PRECISION_C128 or PTE_PRECISION_C64");
     require(cfg->chi_cap >= 1, PTE_ERR_ARG, "chi_cap must be positive");
     require(cfg->n_north >= 1, PTE_ERR_ARG, "n_north must be positive");
     auto* e = new pte_pt_engine();
@@ -1526,8 +1549,8 @@ int pte_tempo_create(const pte_tempo_config* cfg, pte_tempo_engine** out) {
     require(cfg->dkmax >= -1, PTE_ERR_ARG, "dkmax must be >= 0 or -1");
     require(cfg->epsrel > 0.0 && std::isfinite(cfg->epsrel), PTE_ERR_ARG,
             "epsrel must be positive");
-    require(cfg->precision == PTE_PRECISION_C128, PTE_ERR_ARG,
-            "only the complex128 precision is available");
+    require(cfg->precision == PTE_PRECISION_C128 || cfg->precision == PTE_PRECISION_C64,
+            PTE_ERR_ARG, "precision must be PTE_PRECISION_C128 or PTE_PRECISION_C64");
     require(cfg->chi_cap >= 1 && cfg->n_north >= 1 && cfg->n_west >= 1, PTE_ERR_ARG,
             "bad capacity or vector lengths");
     auto* e = new pte_tempo_engine();
@@ -1601,6 +1624,35 @@ void pte_tempo_destroy(pte_tempo_engine* e) {
   if (!e) return;
   guarded([&] { e->impl.destroy(); });
   delete e;
+}
+
+int pte_contract(int precision, int m, int n, int k, const double* A, const double* B,
+                 double* C) {
+  return guarded([&] {
+    require(m >= 1 && n >= 1 && k >= 1 && A && B && C, PTE_ERR_ARG, "bad arguments");
+    require(precision == PTE_PRECISION_C128 || precision == PTE_PRECISION_C64, PTE_ERR_ARG,
+            "unknown precision");
+    Workspace ws;
+    ws.init(1e-7, 1);
+    ws.precision = precision;
+    try {
+      DTensor da = dupload(to_cplx(A, (long)m * k), m, k, 1, 1, ws.stream);
+      DTensor db = dupload(to_cplx(B, (long)k * n), k, n, 1, 1, ws.stream);
+      DTensor dc = dalloc(m, n, 1, 1, ws.stream);
+      ws.contract(m, n, k, da.p, k, 1, db.p, n, 1, dc.p, n, 1);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(C, dc.p, (size_t)m * n * sizeof(double2),
+                               hipMemcpyDeviceToHost, ws.stream));
+      HIP_CHECK(hipStreamSynchronize(ws.stream));
+      dfree(da, ws.stream);
+      dfree(db, ws.stream);
+      dfree(dc, ws.stream);
+    } catch (...) {
+      ws.destroy();
+      throw;
+    }
+    ws.destroy();
+  });
 }
 
 int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap, double* U,

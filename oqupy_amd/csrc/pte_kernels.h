@@ -103,6 +103,7 @@ __global__ void build_rh_kernel(const double2* A, long lda, int n, const int* pi
                                 const double* beta, const int* rank, double2* Y);
 __global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);
+__global__ void cgemm_mfma_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);
 __global__ void mpo_apply_right_kernel(MpoApplyArgs a);
 __global__ void jacobi_kernel(JacobiArgs a);

@@ -113,6 +113,95 @@ zgemm_kernel(GemmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// complex GEMM in complex64 with FP32 accumulation on the matrix cores
+// (precision form): C(i,j) = sum_k A(i,k) B(k,j), inputs and output stored
+// as complex128 and rounded to complex64 when staged. 64x64 output tile per
+// 256-thread workgroup; each wavefront owns a 32x32 quarter as 2x2 tiles of
+// v_mfma_f32_16x16x4_f32, four real products per complex product:
+//   Re += Ar Br - Ai Bi,   Im += Ar Bi + Ai Br.
+// Operand layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; it holds D[4*(l/16)+r][l%16], r = 0..3.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+cgemm_mfma_kernel(GemmArgs a) {
+  __shared__ float2 As[GB_K][GB_M + 1];
+  __shared__ float2 Bs[GB_K][GB_N + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int i0 = blockIdx.y * GB_M, j0 = blockIdx.x * GB_N;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  floatx4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      accr[u][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+      acci[u][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+  const bool a_kfast = (a.sak == 1);
+  const bool b_jfast = (a.sbj == 1);
+  for (int k0 = 0; k0 < a.K; k0 += GB_K) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int row, kk;
+      if (a_kfast) { row = e >> 4; kk = e & 15; } else { row = e & 63; kk = e >> 6; }
+      const int gi = i0 + row, gk = k0 + kk;
+      float2 va = make_float2(0.f, 0.f);
+      if (gi < a.M && gk < a.K) {
+        const double2 v = a.A[(long)gi * a.sai + (long)gk * a.sak];
+        va = make_float2((float)v.x, a.conjA ? -(float)v.y : (float)v.y);
+      }
+      As[kk][row] = va;
+      int col;
+      if (b_jfast) { kk = e >> 6; col = e & 63; } else { kk = e & 15; col = e >> 4; }
+      const int gj = j0 + col, gk2 = k0 + kk;
+      float2 vb = make_float2(0.f, 0.f);
+      if (gj < a.N && gk2 < a.K) {
+        const double2 v = a.B[(long)gk2 * a.sbk + (long)gj * a.sbj];
+        vb = make_float2((float)v.x, a.conjB ? -(float)v.y : (float)v.y);
+      }
+      Bs[kk][col] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < GB_K; k4 += 4) {
+      float2 av[2], bv[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) av[u] = As[k4 + lk][wr + 16 * u + lr];
+#pragma unroll
+      for (int v = 0; v < 2; ++v) bv[v] = Bs[k4 + lk][wc + 16 * v + lr];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          accr[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].x, bv[v].x, accr[u][v], 0, 0, 0);
+          accr[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[u].y, bv[v].y, accr[u][v], 0, 0, 0);
+          acci[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].x, bv[v].y, acci[u][v], 0, 0, 0);
+          acci[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].y, bv[v].x, acci[u][v], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * u + 4 * lk + r;
+        const int gj = j0 + wc + 16 * v + lr;
+        if (gi < a.M && gj < a.N) {
+          double2 c = make_double2((double)accr[u][v][r], (double)acci[u][v][r]);
+          if (!a.beta_zero) c = cadd(c, a.C[(long)gi * a.sci + (long)gj * a.scj]);
+          a.C[(long)gi * a.sci + (long)gj * a.scj] = c;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // T[x,s,l,w] = sum_{p,v} W[w,p,s,v] * G[(l,p),(x,v)]
 //   mode MPO_OUT_SVD_H : X[(l*Wl+w) + (x*S+s)*ldx] = conj(T)   (X = M^H)
 //   mode MPO_OUT_SITE  : out[(l*S+s)*Dx + x] = T   (requires Wl == 1)
