@@ -35,6 +35,7 @@ extern "C" {
                                  singular values and the epsrel cut in FP64   */
 
 typedef struct pte_pt_engine pte_pt_engine;
+typedef struct pte_pt_batch pte_pt_batch;
 typedef struct pte_tempo_engine pte_tempo_engine;
 
 /* Configuration of a PT-TEMPO computation.
@@ -109,6 +110,23 @@ int pte_pt_svd_kernel_stats(pte_pt_engine* e, long* count, double* ms,
                             double* flops, double* bytes);
 
 void pte_pt_destroy(pte_pt_engine* e);
+
+/* ---- batched PT-TEMPO (no counterpart in the reference) --------------------
+ * Many independent PT-TEMPO engines (each created, given its influences and
+ * initialised as above) stepped in lockstep: every truncated SVD of every
+ * engine of one step is served by batched cooperative launches, so that
+ * hundreds of instances with different baths, memory cuts and bond
+ * dimensions share the GPU. Equivalent to calling pte_pt_step(e, NULL, 0, 0,
+ * ...) on each engine (PtTempo.compute, oqupy/pt_tempo.py:258-278, for
+ * baths without a per-step replacement influence); the batch does not own
+ * the engines. */
+int pte_pt_batch_create(pte_pt_engine* const* engines, int count, pte_pt_batch** out);
+
+/* One step of every engine that has steps left; more[i] (count entries)
+ * receives 1 while engine i has steps left. */
+int pte_pt_batch_step(pte_pt_batch* b, int* more);
+
+void pte_pt_batch_destroy(pte_pt_batch* b);
 
 /* ---- TEMPO (oqupy/backends/tempo_backend.py:314-626) ---------------------- */
 

@@ -13,7 +13,7 @@ from oqupy_amd.bath_correlations import (CustomCorrelations, CustomSD,
 from oqupy_amd.dynamics import Dynamics
 from oqupy_amd.process_tensor import (SimpleProcessTensor,
                                       TrivialProcessTensor)
-from oqupy_amd.pt_tempo import PtTempo, pt_tempo_compute
+from oqupy_amd.pt_tempo import PtTempo, pt_tempo_compute, pt_tempo_compute_batch
 from oqupy_amd.system import System, TimeDependentSystem
 from oqupy_amd.system_dynamics import compute_dynamics
 from oqupy_amd.tempo import (Tempo, TempoParameters, influence_matrix,
@@ -24,6 +24,7 @@ __all__ = [
     'Bath', 'CustomCorrelations', 'CustomSD', 'Dynamics', 'PowerLawSD',
     'PtTempo', 'SimpleProcessTensor', 'System', 'Tempo', 'TempoParameters',
     'TimeDependentSystem', 'TrivialProcessTensor', 'compute_dynamics',
-    'influence_matrix', 'operators', 'pt_tempo_compute', 'tempo_compute',
+    'influence_matrix', 'operators', 'pt_tempo_compute',
+    'pt_tempo_compute_batch', 'tempo_compute',
     '__version__',
 ]

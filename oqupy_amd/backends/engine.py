@@ -100,6 +100,10 @@ _SIGNATURES = {
                                      _DP, _IP, _IP]),
     "pte_contract": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _DP, _DP, _DP]),
+    "pte_pt_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int,
+                                           ctypes.POINTER(_P)]),
+    "pte_pt_batch_step": (ctypes.c_int, [_P, _IP]),
+    "pte_pt_batch_destroy": (None, [_P]),
 }
 
 PRECISIONS = {"c128": 0, "c64": 1}

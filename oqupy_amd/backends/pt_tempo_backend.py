@@ -112,6 +112,12 @@ class PtTempoBackend:
         self.step_stats.append(tuple(s.value for s in stats))
         return bool(more.value)
 
+    def sync_step(self) -> None:
+        """Read the step counter back from the engine (after batched steps)."""
+        st = ctypes.c_int()
+        eng.check(self._lib.pte_pt_get_step(self._handle, ctypes.byref(st)))
+        self._step = st.value
+
     def svd_kernel_stats(self) -> dict:
         """Accumulated launches, device milliseconds, algorithmic flops and
         bytes of the Jacobi SVD kernel."""

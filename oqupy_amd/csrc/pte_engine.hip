@@ -17,6 +17,10 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <exception>
+#include <thread>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 
@@ -170,8 +174,15 @@ constexpr int kQrMaxRows = 3584;
 // largest rank a truncated SVD of an m x n matrix can keep
 static inline int k_bound(int m, int n) { return std::max(1, std::min(m, n)); }
 
+struct Batcher;
+
 struct Workspace {
   hipStream_t stream = nullptr;
+  // stream of the SVD phases: `stream`, or the batch stream while a batched
+  // launch serves this workspace
+  hipStream_t svd_stream = nullptr;
+  Batcher* batcher = nullptr;
+  hipEvent_t ev_ready = nullptr;
   DevBuf G, X, V, carry, carry2, svh, gpart, Q;
   // pivoted-QR state of the last svd_qrcp call
   double2* qr_tau = nullptr;
@@ -221,6 +232,8 @@ struct Workspace {
     epsrel = eps;
     chi_cap = cap;
     HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    svd_stream = stream;
+    HIP_CHECK(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     hipMemPool_t pool;
     int dev = 0;
     HIP_CHECK(hipGetDevice(&dev));
@@ -276,6 +289,7 @@ struct Workspace {
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(evj0); (void)hipEventDestroy(evj1);
     (void)hipEventDestroy(evq0); (void)hipEventDestroy(evq1);
+    (void)hipEventDestroy(ev_ready);
     if (trace) std::fclose(trace);
     trace = nullptr;
     (void)hipStreamDestroy(stream);
@@ -285,6 +299,7 @@ struct Workspace {
   void ensure_sv(size_t n) {
     if (n <= nsv_cap) return;
     HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
     size_t c = std::max<size_t>(n, 1024);
     HIP_CHECK(hipMalloc((void**)&sigma, c * sizeof(double)));
@@ -293,30 +308,26 @@ struct Workspace {
     nsv_cap = c;
   }
 
-  // Jacobi SVD of X (m x n, column-major, ld = m, already in X) with V (n x n)
-  // accumulated; sorts and truncates. Returns the kept rank.
-  int svd(int m, int n) {
+  // ---- phases of the truncated SVD, shared by the single and the batched
+  // launches (svd_stream is the stream they are enqueued on) ----
+
+  // Buffers and arguments of the Jacobi on X (m x n, column-major, ld = m);
+  // at most max_grid workgroups. *grid_out receives the grid size.
+  JacobiArgs jacobi_prepare(int m, int n, int max_grid, int* grid_out) {
     require(m >= 1 && n >= 1, PTE_ERR_ARG, "internal: empty SVD");
-    V.ensure((size_t)n * n, stream);
+    V.ensure((size_t)n * n, svd_stream);
     ensure_sv((size_t)n);
-    int nb = (n + kJacobiBlock - 1) / kJacobiBlock;
-    if (nb % 2) ++nb;
-    if (nb < 2) nb = 2;
+    const int nb = jacobi_blocks(n);
     const int npairs = nb / 2;
-    require(npairs <= max_coop_blocks, PTE_ERR_CAPACITY,
+    require(npairs <= max_grid, PTE_ERR_CAPACITY,
             "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
     // row slices per block pair: each slice keeps at least slice_rows rows of
-    // X, at most 16 slices, within the co-resident grid and the barrier's
-    // 1024 workgroups
-    const int R = std::max(1, std::min({std::min(max_coop_blocks, 1024) / npairs,
+    // X, at most 16 slices, within the grid
+    const int R = std::max(1, std::min({max_grid / npairs,
                                         (m + slice_rows - 1) / slice_rows, 16}));
-    const int grid = npairs * R;
     auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
     const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
-    if (R > 1) gpart.ensure((size_t)npairs * R * 1024, stream);
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, stream));
-    HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, stream));
-    HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), stream));
+    if (R > 1) gpart.ensure((size_t)npairs * R * 1024, svd_stream);
     JacobiArgs ja;
     ja.X = X.p; ja.m = m; ja.n = n; ja.ldx = m;
     ja.V = V.p; ja.ldv = n;
@@ -329,18 +340,42 @@ struct Workspace {
     ja.inner_sweeps = inner_sweeps;
     ja.bar = bar; ja.flags = flags; ja.fro2 = fro2;
     ja.info = &info_d->sweeps;
-    void* kargs[] = {&ja};
-    HIP_CHECK(hipEventRecord(evj0, stream));
-    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
-                                         dim3(grid), dim3(256), kargs, 0, stream));
-    HIP_CHECK(hipEventRecord(evj1, stream));
-    column_norm_kernel<<<n, 256, 0, stream>>>(X.p, m, n, m, sigma);
-    rank_sort_kernel<<<(n + 255) / 256, 256, 0, stream>>>(sigma, n, perm, sorted);
-    truncate_kernel<<<1, 1024, 0, stream>>>(sorted, n, epsrel, info_d, chi_cap);
+    *grid_out = npairs * R;
+    return ja;
+  }
+
+  static int jacobi_blocks(int n) {
+    int nb = (n + kJacobiBlock - 1) / kJacobiBlock;
+    if (nb % 2) ++nb;
+    return std::max(nb, 2);
+  }
+
+  // natural grid of a Jacobi on m x n (before any budget)
+  int jacobi_natural_grid(int m, int n) const {
+    const int npairs = jacobi_blocks(n) / 2;
+    const int R = std::max(1, std::min({std::min(max_coop_blocks, 1024) / std::max(npairs, 1),
+                                        (m + slice_rows - 1) / slice_rows, 16}));
+    return npairs * R;
+  }
+
+  void jacobi_reset() {
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
+    HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, svd_stream));
+    HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), svd_stream));
+  }
+
+  // singular values, descending order, cut; the summary goes to info_h
+  void jacobi_finish(int m, int n) {
+    column_norm_kernel<<<n, 256, 0, svd_stream>>>(X.p, m, n, m, sigma);
+    rank_sort_kernel<<<(n + 255) / 256, 256, 0, svd_stream>>>(sigma, n, perm, sorted);
+    truncate_kernel<<<1, 1024, 0, svd_stream>>>(sorted, n, epsrel, info_d, chi_cap);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
-                             stream));
-    HIP_CHECK(hipStreamSynchronize(stream));
+                             svd_stream));
+  }
+
+  // after synchronisation: checks, statistics, kept rank
+  int jacobi_check(int m, int n, float ms) {
     require(info_h->sweeps > 0, PTE_ERR_NUMERIC,
             "Jacobi SVD did not converge (" + std::to_string(m) + "x" +
                 std::to_string(n) + ")");
@@ -348,23 +383,19 @@ struct Workspace {
             "bond dimension " + std::to_string(info_h->rank) +
                 " exceeds the capacity chi_cap=" + std::to_string(chi_cap));
     require(info_h->rank >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
-    {
-      float ms = 0.f;
-      HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
-      // algorithmic work of a thin complex SVD of an M x N matrix (R-SVD
-      // count of Golub & Van Loan, times 4 for complex arithmetic) and the
-      // HBM bytes of reading and writing M once plus writing V
-      const double mm = std::max(m, n), nn = std::min(m, n);
-      jac_ms += ms;
-      jac_count += 1;
-      jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
-      jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
-      if (trace) {
-        if (tr_r == 0) { tr_m = n; tr_n = m; tr_qr_ms = 0.f; }
-        std::fprintf(trace, "%d,%d,%d,%d,%d,%.4f,%.4f\n", tr_m, tr_n, tr_r, info_h->rank,
-                     info_h->sweeps, tr_qr_ms, ms);
-        tr_r = 0;
-      }
+    // algorithmic work of a thin complex SVD of an M x N matrix (R-SVD count
+    // of Golub & Van Loan, times 4 for complex arithmetic) and the HBM bytes
+    // of reading and writing M once plus writing V
+    const double mm = std::max(m, n), nn = std::min(m, n);
+    jac_ms += ms;
+    jac_count += 1;
+    jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
+    jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
+    if (trace) {
+      if (tr_r == 0) { tr_m = n; tr_n = m; tr_qr_ms = 0.f; }
+      std::fprintf(trace, "%d,%d,%d,%d,%d,%.4f,%.4f\n", tr_m, tr_n, tr_r, info_h->rank,
+                   info_h->sweeps, tr_qr_ms, ms);
+      tr_r = 0;
     }
     n_svd += 1;
     sweeps += info_h->sweeps;
@@ -372,9 +403,29 @@ struct Workspace {
     return info_h->rank;
   }
 
+  // Jacobi SVD of X (m x n, column-major, ld = m, already in X) with V (n x n)
+  // accumulated; sorts and truncates. Returns the kept rank.
+  int svd(int m, int n) {
+    if (batcher) return batched_svd(false, m, n);
+    int grid = 0;
+    JacobiArgs ja = jacobi_prepare(m, n, std::min(max_coop_blocks, 1024), &grid);
+    jacobi_reset();
+    void* kargs[] = {&ja};
+    HIP_CHECK(hipEventRecord(evj0, svd_stream));
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
+                                         dim3(grid), dim3(256), kargs, 0, svd_stream));
+    HIP_CHECK(hipEventRecord(evj1, svd_stream));
+    jacobi_finish(m, n);
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
+    return jacobi_check(m, n, ms);
+  }
+
   void ensure_qr(size_t n) {
     if (n <= qr_cap) return;
     HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
     (void)hipFree(qr_tau); (void)hipFree(qr_vscale); (void)hipFree(qr_beta);
     (void)hipFree(qr_piv); (void)hipFree(qr_done);
     const size_t c = std::max<size_t>(n, 1024);
@@ -394,42 +445,58 @@ struct Workspace {
   //   row j of S V^H at column piv[s] = conj(Z[s, perm[j]])  (emit_svh)
   // Returns the kept rank k.
   int svd_qrcp(int m, int n) {
+    if (batcher) return batched_svd(true, m, n);
+    const int nblk = qr_natural_grid(n);
+    QrcpArgs qa = qr_prepare(m, n, nblk);
+    void* kargs[] = {&qa};
+    HIP_CHECK(hipEventRecord(evq0, svd_stream));
+    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qrcp_kernel),
+                                         dim3(nblk), dim3(256), kargs,
+                                         (size_t)m * sizeof(double2), svd_stream));
+    HIP_CHECK(hipEventRecord(evq1, svd_stream));
+    HIP_CHECK(hipMemcpyAsync(qr_rank_h, qr_rank_d, sizeof(int), hipMemcpyDeviceToHost,
+                             svd_stream));
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, evq0, evq1));
+    const int r = qr_finish(m, n, ms);
+    return svd(n, r);
+  }
+
+  int qr_natural_grid(int n) const { return std::max(1, std::min({n, qr_blocks, num_cus})); }
+  static int qr_min_grid(int n) { return std::max(1, (n + kQrcpMaxLocal - 1) / kQrcpMaxLocal); }
+
+  // buffers and arguments of the pivoted QR of Q (m x n) on nblk workgroups
+  QrcpArgs qr_prepare(int m, int n, int nblk) {
     require(m >= 1 && n >= 1 && m <= kQrMaxRows, PTE_ERR_ARG, "internal: bad QR shape");
     ensure_qr((size_t)n);
-    const int nblk = std::max(1, std::min({n, qr_blocks, num_cus}));
     require((n + nblk - 1) / nblk <= kQrcpMaxLocal, PTE_ERR_CAPACITY,
             "matrix of " + std::to_string(n) + " columns exceeds the pivoted QR");
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, stream));
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
     QrcpArgs qa;
     qa.A = Q.p; qa.m = m; qa.n = n; qa.lda = m;
     qa.drop = 2.220446049250313e-16;
     qa.tau = qr_tau; qa.vscale = qr_vscale; qa.beta = qr_beta;
     qa.piv = qr_piv; qa.done = qr_done; qa.cand = qr_cand; qa.rank = qr_rank_d;
     qa.bar = bar;
-    void* kargs[] = {&qa};
-    HIP_CHECK(hipEventRecord(evq0, stream));
-    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qrcp_kernel),
-                                         dim3(nblk), dim3(256), kargs,
-                                         (size_t)m * sizeof(double2), stream));
-    HIP_CHECK(hipEventRecord(evq1, stream));
-    HIP_CHECK(hipMemcpyAsync(qr_rank_h, qr_rank_d, sizeof(int), hipMemcpyDeviceToHost,
-                             stream));
-    HIP_CHECK(hipStreamSynchronize(stream));
+    return qa;
+  }
+
+  // after synchronisation: rank of the pivoted QR, Y = R^H into X
+  int qr_finish(int m, int n, float ms) {
     const int r = *qr_rank_h;
     require(r >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
-    {
-      float ms = 0.f;
-      HIP_CHECK(hipEventElapsedTime(&ms, evq0, evq1));
-      qr_ms += ms;
-      qr_count += 1;
-      tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = ms;
-    }
-    X.ensure((size_t)n * r, stream);
-    build_rh_kernel<<<grid_for((long)n * r), 256, 0, stream>>>(Q.p, m, n, qr_piv, qr_beta,
-                                                               qr_rank_d, X.p);
+    qr_ms += ms;
+    qr_count += 1;
+    tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = ms;
+    X.ensure((size_t)n * r, svd_stream);
+    build_rh_kernel<<<grid_for((long)n * r), 256, 0, svd_stream>>>(Q.p, m, n, qr_piv,
+                                                                   qr_beta, qr_rank_d, X.p);
     qr_m = m; qr_n = n; qr_r = r;
-    return svd(n, r);
+    return r;
   }
+
+  int batched_svd(bool qr, int m, int n);
 
   // dst[j*dj + a*da + b*db] = (Q W[:, perm[j]])[a*B + b] for j < k, with
   // every stride d = d0 + dk * k
@@ -497,6 +564,319 @@ struct Workspace {
     cgemm_mfma_kernel<<<grid, 256, 0, stream>>>(g);
   }
 };
+
+// ---------------------------------------------------------------------------
+// Batched truncated SVDs for many engines stepped together (one host thread
+// per engine). An engine thread that reaches an SVD posts a request and
+// waits; when every running thread has posted (or left), the last one to
+// arrive serves all requests at once on the batch stream: one cooperative
+// launch of qrcp_batch_kernel for the pivoted QRs, one of
+// jacobi_batch_kernel for the Jacobi SVDs, each workgroup working on one
+// request with that request's own barrier counter. Requests that do not fit
+// the co-resident grid together go into several launches.
+// ---------------------------------------------------------------------------
+struct SvdRequest {
+  Workspace* ws = nullptr;
+  bool qr = false;
+  int m = 0, n = 0;       // shape handed to the SVD (Q for qr, X otherwise)
+  int jm = 0, jn = 0;     // shape of the Jacobi
+  int k = 0;
+  bool done = false;
+  bool failed = false;
+  std::exception_ptr err;
+};
+
+struct Batcher {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex mu;
+  std::condition_variable cv;
+  int running = 0;
+  std::vector<SvdRequest*> pending;
+  int max_jacobi_grid = 0;
+  int num_cus = 0;
+  // argument arrays: pinned staging and device copies
+  char* stage_h = nullptr;
+  char* stage_d = nullptr;
+  size_t stage_cap = 0;
+
+  void init() {
+    HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+    int dev = 0, per_cu = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    num_cus = prop.multiProcessorCount;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(jacobi_batch_kernel), 256, 0));
+    max_jacobi_grid = std::min(per_cu * num_cus, 1024);
+    require(max_jacobi_grid > 0, PTE_ERR_DEVICE, "batched Jacobi kernel cannot be resident");
+  }
+
+  void destroy() {
+    if (!stream) return;
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(stage_d);
+    (void)hipHostFree(stage_h);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  void ensure_stage(size_t bytes) {
+    if (bytes <= stage_cap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    (void)hipFree(stage_d);
+    (void)hipHostFree(stage_h);
+    const size_t c = std::max<size_t>(bytes, 1 << 16);
+    HIP_CHECK(hipHostMalloc((void**)&stage_h, c, hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void**)&stage_d, c));
+    stage_cap = c;
+  }
+
+  void enter(int count) {
+    std::lock_guard<std::mutex> lk(mu);
+    running += count;
+  }
+
+  // an engine thread is done with its step
+  void leave() {
+    std::unique_lock<std::mutex> lk(mu);
+    running -= 1;
+    serve_if_complete(lk);
+  }
+
+  int submit(Workspace* ws, bool qr, int m, int n) {
+    SvdRequest rq;
+    rq.ws = ws; rq.qr = qr; rq.m = m; rq.n = n;
+    HIP_CHECK(hipEventRecord(ws->ev_ready, ws->stream));
+    std::unique_lock<std::mutex> lk(mu);
+    pending.push_back(&rq);
+    serve_if_complete(lk);
+    cv.wait(lk, [&] { return rq.done; });
+    if (rq.failed) std::rethrow_exception(rq.err);
+    return rq.k;
+  }
+
+  // with the lock held: when every running thread waits, serve them
+  void serve_if_complete(std::unique_lock<std::mutex>& lk) {
+    if (pending.empty() || (int)pending.size() < running) return;
+    std::vector<SvdRequest*> batch;
+    batch.swap(pending);
+    lk.unlock();
+    try {
+      execute(batch);
+    } catch (...) {
+      for (auto* r : batch) {
+        r->ws->svd_stream = r->ws->stream;
+        if (!r->failed) { r->failed = true; r->err = std::current_exception(); }
+      }
+    }
+    lk.lock();
+    for (auto* r : batch) r->done = true;
+    cv.notify_all();
+  }
+
+  // greedy chunks of consecutive requests whose minimum grids fit `cap`
+  template <class MinGrid>
+  static std::vector<std::pair<size_t, size_t>> chunks(const std::vector<SvdRequest*>& rs,
+                                                       int cap, MinGrid min_grid) {
+    std::vector<std::pair<size_t, size_t>> out;
+    size_t i = 0;
+    while (i < rs.size()) {
+      size_t j = i;
+      long used = 0;
+      while (j < rs.size() && (j == i || used + min_grid(rs[j]) <= cap)) {
+        used += min_grid(rs[j]);
+        ++j;
+      }
+      out.emplace_back(i, j);
+      i = j;
+    }
+    return out;
+  }
+
+  // copy the per-request arguments, starts and the workgroup map to the device
+  template <class Args>
+  void upload(const std::vector<Args>& args, const std::vector<int>& grids, Args** dargs,
+              int** dstarts, int** dmap, int* total) {
+    const size_t na = args.size();
+    int t = 0;
+    for (int g : grids) t += g;
+    *total = t;
+    const size_t b_args = na * sizeof(Args);
+    const size_t o_starts = (b_args + 255) / 256 * 256;
+    const size_t o_map = o_starts + ((na + 1) * sizeof(int) + 255) / 256 * 256;
+    const size_t bytes = o_map + (size_t)t * sizeof(int);
+    ensure_stage(bytes);
+    HIP_CHECK(hipStreamSynchronize(stream));   // the staging buffer is free again
+    std::memcpy(stage_h, args.data(), b_args);
+    int* st = reinterpret_cast<int*>(stage_h + o_starts);
+    int* mp = reinterpret_cast<int*>(stage_h + o_map);
+    st[0] = 0;
+    for (size_t p = 0; p < na; ++p) {
+      st[p + 1] = st[p] + grids[p];
+      for (int b = st[p]; b < st[p + 1]; ++b) mp[b] = (int)p;
+    }
+    HIP_CHECK(hipMemcpyAsync(stage_d, stage_h, bytes, hipMemcpyHostToDevice, stream));
+    *dargs = reinterpret_cast<Args*>(stage_d);
+    *dstarts = reinterpret_cast<int*>(stage_d + o_starts);
+    *dmap = reinterpret_cast<int*>(stage_d + o_map);
+  }
+
+  // workgroups per request: its minimum, plus a share of what is left of
+  // `cap` proportional to what it could still use (sum <= cap when the
+  // minima fit)
+  static std::vector<int> allocate(const std::vector<int>& nat, const std::vector<int>& mn,
+                                   int cap) {
+    long smin = 0, sextra = 0;
+    for (size_t i = 0; i < nat.size(); ++i) {
+      smin += mn[i];
+      sextra += std::max(0, nat[i] - mn[i]);
+    }
+    const long rem = std::max(0L, (long)cap - smin);
+    std::vector<int> g(nat.size());
+    for (size_t i = 0; i < nat.size(); ++i) {
+      const long extra = std::max(0, nat[i] - mn[i]);
+      const long add = (sextra <= rem) ? extra : (sextra > 0 ? extra * rem / sextra : 0);
+      g[i] = mn[i] + (int)add;
+    }
+    return g;
+  }
+
+  static void fail(SvdRequest* r) {
+    r->failed = true;
+    r->err = std::current_exception();
+  }
+
+  void execute(std::vector<SvdRequest*>& batch) {
+    for (auto* r : batch) {
+      HIP_CHECK(hipStreamWaitEvent(stream, r->ws->ev_ready, 0));
+      r->ws->svd_stream = stream;
+    }
+    // ---- pivoted QRs ----
+    std::vector<SvdRequest*> qrs;
+    for (auto* r : batch)
+      if (r->qr) qrs.push_back(r);
+    if (!qrs.empty()) {
+      int max_m = 0;
+      for (auto* r : qrs) max_m = std::max(max_m, r->m);
+      const size_t lds = (size_t)max_m * sizeof(double2);
+      int per_cu = 0;
+      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(qrcp_batch_kernel), 256, lds));
+      const int cap = std::min(per_cu * num_cus, 1024);
+      require(cap > 0, PTE_ERR_DEVICE, "batched QR kernel cannot be resident");
+      auto ming = [](SvdRequest* r) { return Workspace::qr_min_grid(r->n); };
+      std::vector<SvdRequest*> ok;
+      for (auto& ch : chunks(qrs, cap, ming)) {
+        std::vector<int> nat, mn;
+        for (size_t i = ch.first; i < ch.second; ++i) {
+          nat.push_back(qrs[i]->ws->qr_natural_grid(qrs[i]->n));
+          mn.push_back(Workspace::qr_min_grid(qrs[i]->n));
+        }
+        const std::vector<int> alloc = allocate(nat, mn, cap);
+        std::vector<QrcpArgs> args;
+        std::vector<int> grids;
+        std::vector<SvdRequest*> in;
+        for (size_t i = ch.first; i < ch.second; ++i) {
+          SvdRequest* r = qrs[i];
+          try {
+            const int g = alloc[i - ch.first];
+            args.push_back(r->ws->qr_prepare(r->m, r->n, g));
+            grids.push_back(g);
+            in.push_back(r);
+          } catch (...) {
+            fail(r);
+          }
+        }
+        if (in.empty()) continue;
+        QrcpArgs* da; int* ds; int* dm; int total = 0;
+        upload(args, grids, &da, &ds, &dm, &total);
+        require(total <= cap, PTE_ERR_CAPACITY, "internal: batched QR grid too large");
+        void* kargs[] = {&da, &ds, &dm};
+        HIP_CHECK(hipEventRecord(ev0, stream));
+        HIP_CHECK(hipLaunchCooperativeKernel(
+            reinterpret_cast<const void*>(qrcp_batch_kernel), dim3(total), dim3(256), kargs,
+            lds, stream));
+        HIP_CHECK(hipEventRecord(ev1, stream));
+        for (auto* r : in)
+          HIP_CHECK(hipMemcpyAsync(r->ws->qr_rank_h, r->ws->qr_rank_d, sizeof(int),
+                                   hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        for (auto* r : in) {
+          try {
+            const int rk = r->ws->qr_finish(r->m, r->n, ms);
+            r->jm = r->n;
+            r->jn = rk;
+          } catch (...) {
+            fail(r);
+          }
+        }
+      }
+    }
+    for (auto* r : batch)
+      if (!r->qr) { r->jm = r->m; r->jn = r->n; }
+    // ---- Jacobi SVDs ----
+    std::vector<SvdRequest*> jr;
+    for (auto* r : batch)
+      if (!r->failed) jr.push_back(r);
+    const int cap = max_jacobi_grid;
+    auto ming = [](SvdRequest* r) { return Workspace::jacobi_blocks(r->jn) / 2; };
+    for (auto& ch : chunks(jr, cap, ming)) {
+      std::vector<int> nat, mn;
+      for (size_t i = ch.first; i < ch.second; ++i) {
+        nat.push_back(jr[i]->ws->jacobi_natural_grid(jr[i]->jm, jr[i]->jn));
+        mn.push_back(Workspace::jacobi_blocks(jr[i]->jn) / 2);
+      }
+      const std::vector<int> alloc = allocate(nat, mn, cap);
+      std::vector<JacobiArgs> args;
+      std::vector<int> grids;
+      std::vector<SvdRequest*> in;
+      for (size_t i = ch.first; i < ch.second; ++i) {
+        SvdRequest* r = jr[i];
+        try {
+          int g = 0;
+          args.push_back(r->ws->jacobi_prepare(r->jm, r->jn, alloc[i - ch.first], &g));
+          r->ws->jacobi_reset();
+          grids.push_back(g);
+          in.push_back(r);
+        } catch (...) {
+          fail(r);
+        }
+      }
+      if (in.empty()) continue;
+      JacobiArgs* da; int* ds; int* dm; int total = 0;
+      upload(args, grids, &da, &ds, &dm, &total);
+      require(total <= cap, PTE_ERR_CAPACITY, "internal: batched Jacobi grid too large");
+      void* kargs[] = {&da, &ds, &dm};
+      HIP_CHECK(hipEventRecord(ev0, stream));
+      HIP_CHECK(hipLaunchCooperativeKernel(
+          reinterpret_cast<const void*>(jacobi_batch_kernel), dim3(total), dim3(256), kargs,
+          0, stream));
+      HIP_CHECK(hipEventRecord(ev1, stream));
+      for (auto* r : in) r->ws->jacobi_finish(r->jm, r->jn);
+      HIP_CHECK(hipStreamSynchronize(stream));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      for (auto* r : in) {
+        try {
+          r->k = r->ws->jacobi_check(r->jm, r->jn, ms);
+        } catch (...) {
+          fail(r);
+        }
+      }
+    }
+    for (auto* r : batch) r->ws->svd_stream = r->ws->stream;
+  }
+};
+
+int Workspace::batched_svd(bool qr, int m, int n) { return batcher->submit(this, qr, m, n); }
 
 // ---------------------------------------------------------------------------
 // chain operations (node_array.py) on device sites [l][p][r]
@@ -1385,6 +1765,10 @@ using namespace pte;
 
 struct pte_pt_engine { PtEngine impl; };
 struct pte_tempo_engine { TempoEngine impl; };
+struct pte_pt_batch {
+  std::vector<pte_pt_engine*> engines;
+  Batcher batcher;
+};
 
 static void store_infl(HostInfl& h, const double* data, int rows, int cols) {
   require(data != nullptr && rows > 0 && cols >= 0, PTE_ERR_ARG, "bad influence shape");
@@ -1539,6 +1923,69 @@ void pte_pt_destroy(pte_pt_engine* e) {
   if (!e) return;
   guarded([&] { e->impl.destroy(); });
   delete e;
+}
+
+int pte_pt_batch_create(pte_pt_engine* const* engines, int count, pte_pt_batch** out) {
+  return guarded([&] {
+    require(engines && out && count >= 1, PTE_ERR_ARG, "bad batch arguments");
+    for (int i = 0; i < count; ++i) {
+      require(engines[i] != nullptr, PTE_ERR_ARG, "null engine in batch");
+      for (int j = 0; j < i; ++j)
+        require(engines[j] != engines[i], PTE_ERR_ARG, "engine listed twice in batch");
+    }
+    auto* b = new pte_pt_batch();
+    b->engines.assign(engines, engines + count);
+    try {
+      b->batcher.init();
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+int pte_pt_batch_step(pte_pt_batch* b, int* more) {
+  return guarded([&] {
+    require(b != nullptr && more != nullptr, PTE_ERR_ARG, "null argument");
+    std::vector<PtEngine*> active;
+    for (auto* e : b->engines) {
+      require(e->impl.step >= 1, PTE_ERR_STATE, "engine in batch not initialised");
+      if (e->impl.step < e->impl.cfg.num_steps) active.push_back(&e->impl);
+    }
+    std::vector<std::exception_ptr> errs(active.size());
+    if (!active.empty()) {
+      b->batcher.enter((int)active.size());
+      std::vector<std::thread> threads;
+      threads.reserve(active.size());
+      for (size_t i = 0; i < active.size(); ++i) {
+        threads.emplace_back([&, i] {
+          PtEngine* eng = active[i];
+          eng->ws.batcher = &b->batcher;
+          try {
+            eng->compute_step(nullptr, 0, 0);
+          } catch (...) {
+            errs[i] = std::current_exception();
+          }
+          eng->ws.batcher = nullptr;
+          b->batcher.leave();
+        });
+      }
+      for (auto& t : threads) t.join();
+    }
+    for (size_t i = 0; i < b->engines.size(); ++i) {
+      const PtEngine& e = b->engines[i]->impl;
+      more[i] = e.step < e.cfg.num_steps ? 1 : 0;
+    }
+    for (auto& er : errs)
+      if (er) std::rethrow_exception(er);
+  });
+}
+
+void pte_pt_batch_destroy(pte_pt_batch* b) {
+  if (!b) return;
+  guarded([&] { b->batcher.destroy(); });
+  delete b;
 }
 
 int pte_tempo_create(const pte_tempo_config* cfg, pte_tempo_engine** out) {

@@ -99,6 +99,8 @@ struct ApplyQArgs {
 constexpr int kQrcpMaxLocal = 64;         // columns per workgroup in qrcp_kernel
 
 __global__ void qrcp_kernel(QrcpArgs a);
+__global__ void qrcp_batch_kernel(const QrcpArgs* probs, const int* starts,
+                                  const int* block_prob);
 __global__ void build_rh_kernel(const double2* A, long lda, int n, const int* piv,
                                 const double* beta, const int* rank, double2* Y);
 __global__ void apply_q_kernel(ApplyQArgs a);
@@ -107,6 +109,8 @@ __global__ void cgemm_mfma_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);
 __global__ void mpo_apply_right_kernel(MpoApplyArgs a);
 __global__ void jacobi_kernel(JacobiArgs a);
+__global__ void jacobi_batch_kernel(const JacobiArgs* probs, const int* starts,
+                                    const int* block_prob);
 __global__ void column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma);
 __global__ void rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted);
 __global__ void truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,

@@ -348,8 +348,8 @@ __device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 
   return true;
 }
 
-__global__ void __launch_bounds__(JT)
-jacobi_kernel(JacobiArgs a) {
+__device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
+                                            const unsigned nblk) {
   __shared__ double2 Xs[JR][JP + 1];
   __shared__ double2 Gs[2][JP][JP + 1];
   __shared__ double2 Ws[JP][JP + 1];
@@ -358,21 +358,20 @@ jacobi_kernel(JacobiArgs a) {
   __shared__ double red[JT];
 
   const int tid = threadIdx.x;
-  const unsigned nblk = gridDim.x;
   const int R = a.slices;
-  const int pair = blockIdx.x / R, slice = blockIdx.x % R;
+  const int pair = bid / R, slice = bid % R;
   const int x_lo = min(a.m, slice * a.xrows_per), x_hi = min(a.m, x_lo + a.xrows_per);
   const int v_lo = min(a.n, slice * a.vrows_per), v_hi = min(a.n, v_lo + a.vrows_per);
   unsigned epoch = 0;
 
   // V = identity, Frobenius norm of X
-  for (long e = blockIdx.x * (long)JT + tid; e < (long)a.n * a.n;
+  for (long e = bid * (long)JT + tid; e < (long)a.n * a.n;
        e += (long)nblk * JT) {
     const long i = e % a.n, j = e / a.n;
     st_agent(a.V + (i + j * a.ldv), make_double2(i == j ? 1.0 : 0.0, 0.0));
   }
   double part = 0.0;
-  for (long e = blockIdx.x * (long)JT + tid; e < (long)a.m * a.n;
+  for (long e = bid * (long)JT + tid; e < (long)a.m * a.n;
        e += (long)nblk * JT) {
     const long i = e % a.m, j = e / a.m;
     part += cabs2(a.X[i + j * a.ldx]);
@@ -594,7 +593,23 @@ jacobi_kernel(JacobiArgs a) {
       break;
     }
   }
-  if (blockIdx.x == 0 && tid == 0) *a.info = converged ? sweep + 1 : -1;
+  if (bid == 0 && tid == 0) *a.info = converged ? sweep + 1 : -1;
+}
+
+__global__ void __launch_bounds__(JT)
+jacobi_kernel(JacobiArgs a) {
+  jacobi_body(a, blockIdx.x, gridDim.x);
+}
+
+// Several independent SVDs in one cooperative launch: workgroup b works on
+// problem block_prob[b] as its (b - starts[p])-th workgroup of
+// starts[p+1] - starts[p]; every problem has its own barrier counter.
+__global__ void __launch_bounds__(JT)
+jacobi_batch_kernel(const JacobiArgs* probs, const int* starts, const int* block_prob) {
+  const int p = block_prob[blockIdx.x];
+  const int s0 = starts[p];
+  const JacobiArgs a = probs[p];
+  jacobi_body(a, blockIdx.x - s0, starts[p + 1] - s0);
 }
 
 // sigma_j = || X[:, j] ||  (one workgroup per column)
@@ -777,8 +792,7 @@ __device__ __forceinline__ void qrcp_post(QrcpCand* dst, double wmax, int wcol, 
   }
 }
 
-__global__ void __launch_bounds__(256)
-qrcp_kernel(QrcpArgs a) {
+__device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, const int nblk) {
   extern __shared__ double2 vcol[];            // reflector of the step (m)
   __shared__ double slot[2][4], slotv[2][4];
   __shared__ int slotc[2][4];
@@ -787,7 +801,7 @@ qrcp_kernel(QrcpArgs a) {
   __shared__ double cw_max[4], cw_sum[4];
   __shared__ int cw_col[4];
   int parity = 0;
-  const int nblk = gridDim.x, blk = blockIdx.x, tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int m = a.m, n = a.n;
   const long lda = a.lda;
@@ -912,6 +926,20 @@ qrcp_kernel(QrcpArgs a) {
       if (!ld_agent(a.done + c)) a.piv[t++] = c;
     *a.rank = j;
   }
+}
+
+__global__ void __launch_bounds__(256)
+qrcp_kernel(QrcpArgs a) {
+  qrcp_body(a, blockIdx.x, gridDim.x);
+}
+
+// Several pivoted QRs in one cooperative launch (see jacobi_batch_kernel).
+__global__ void __launch_bounds__(256)
+qrcp_batch_kernel(const QrcpArgs* probs, const int* starts, const int* block_prob) {
+  const int p = block_prob[blockIdx.x];
+  const int s0 = starts[p];
+  const QrcpArgs a = probs[p];
+  qrcp_body(a, blockIdx.x - s0, starts[p + 1] - s0);
 }
 
 // Y = R^H (n x r, column-major, ld = n) in the pivoted column order:

@@ -6,8 +6,9 @@ non-diagonal coupling, resumable step loop). The steps run in the MI355X
 engine (``oqupy_amd.backends.pt_tempo_backend.PtTempoBackend``).
 """
 
+import ctypes
 from copy import copy
-from typing import Dict, Optional, Text, Union
+from typing import Dict, List, Optional, Text, Union
 
 import numpy as np
 from numpy import ndarray
@@ -168,3 +169,63 @@ def pt_tempo_compute(bath: Bath, start_time: float, end_time: float,
                   description)
     ptt.compute(progress_type=progress_type)
     return ptt.get_process_tensor()
+
+
+def pt_tempo_compute_batch(baths: List[Bath],
+                           start_time: Union[float, List[float]],
+                           end_time: Union[float, List[float]],
+                           parameters: Union[TempoParameters,
+                                             List[TempoParameters]],
+                           unique: Optional[bool] = False,
+                           backend_config: Optional[Dict] = None,
+                           names: Optional[List[Text]] = None
+                           ) -> List[BaseProcessTensor]:
+    """Process tensors of many baths computed together: the engines of all
+    instances are stepped in lockstep and every truncated SVD of one step is
+    served by batched device launches (no counterpart in the reference; each
+    result equals ``pt_tempo_compute(bath_i, start_i, end_i, parameters_i)``,
+    oqupy/pt_tempo.py:309-375). ``start_time``, ``end_time`` and
+    ``parameters`` are either one value for all baths or one per bath.
+    Parameters with ``add_correlation_time`` are not supported here."""
+    from oqupy_amd.backends import engine as eng
+    count = len(baths)
+    if count == 0:
+        return []
+
+    def per_bath(value, what):
+        if isinstance(value, (list, tuple)):
+            if len(value) != count:
+                raise ValueError(f"`{what}` needs one entry per bath.")
+            return list(value)
+        return [value] * count
+
+    starts = per_bath(start_time, "start_time")
+    ends = per_bath(end_time, "end_time")
+    params = per_bath(parameters, "parameters")
+    names = per_bath(names, "names") if names is not None else [None] * count
+    for p in params:
+        if p is None:
+            raise NotImplementedError(
+                "TEMPO parameter estimation is not provided; pass "
+                "`parameters`.")
+        if p.add_correlation_time is not None:
+            raise NotImplementedError(
+                "Batched PT-TEMPO does not support `add_correlation_time`.")
+    ptts = [PtTempo(b, s, e, p, unique, None, False, backend_config, n)
+            for b, s, e, p, n in zip(baths, starts, ends, params, names)]
+    backends = [p.backend for p in ptts]
+    for b in backends:
+        b.initialize()
+    lib = eng.load_library()
+    handles = (ctypes.c_void_p * count)(*[b.handle.value for b in backends])
+    batch = ctypes.c_void_p()
+    eng.check(lib.pte_pt_batch_create(handles, count, ctypes.byref(batch)))
+    try:
+        more = np.ones(count, dtype=np.int32)
+        while more.any():
+            eng.check(lib.pte_pt_batch_step(batch, eng.iptr(more)))
+    finally:
+        lib.pte_pt_batch_destroy(batch)
+    for b in backends:
+        b.sync_step()
+    return [p.get_process_tensor() for p in ptts]
