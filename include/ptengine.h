@@ -122,9 +122,11 @@ void pte_pt_destroy(pte_pt_engine* e);
  * the engines. */
 int pte_pt_batch_create(pte_pt_engine* const* engines, int count, pte_pt_batch** out);
 
-/* One step of every engine that has steps left; more[i] (count entries)
+/* Up to max_steps steps of every engine that has steps left, each engine
+ * advancing on its own (no lockstep at step boundaries: whatever SVDs the
+ * engines have reached are served together); more[i] (count entries)
  * receives 1 while engine i has steps left. */
-int pte_pt_batch_step(pte_pt_batch* b, int* more);
+int pte_pt_batch_run(pte_pt_batch* b, int max_steps, int* more);
 
 void pte_pt_batch_destroy(pte_pt_batch* b);
 

@@ -102,7 +102,7 @@ _SIGNATURES = {
                                     ctypes.c_int, _DP, _DP, _DP]),
     "pte_pt_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int,
                                            ctypes.POINTER(_P)]),
-    "pte_pt_batch_step": (ctypes.c_int, [_P, _IP]),
+    "pte_pt_batch_run": (ctypes.c_int, [_P, ctypes.c_int, _IP]),
     "pte_pt_batch_destroy": (None, [_P]),
 }
 

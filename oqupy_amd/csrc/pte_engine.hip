@@ -448,6 +448,7 @@ struct Workspace {
     if (batcher) return batched_svd(true, m, n);
     const int nblk = qr_natural_grid(n);
     QrcpArgs qa = qr_prepare(m, n, nblk);
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
     void* kargs[] = {&qa};
     HIP_CHECK(hipEventRecord(evq0, svd_stream));
     HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qrcp_kernel),
@@ -472,7 +473,6 @@ struct Workspace {
     ensure_qr((size_t)n);
     require((n + nblk - 1) / nblk <= kQrcpMaxLocal, PTE_ERR_CAPACITY,
             "matrix of " + std::to_string(n) + " columns exceeds the pivoted QR");
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
     QrcpArgs qa;
     qa.A = Q.p; qa.m = m; qa.n = n; qa.lda = m;
     qa.drop = 2.220446049250313e-16;
@@ -482,18 +482,35 @@ struct Workspace {
     return qa;
   }
 
-  // after synchronisation: rank of the pivoted QR, Y = R^H into X
-  int qr_finish(int m, int n, float ms) {
-    const int r = *qr_rank_h;
+  // after synchronisation: rank r of the pivoted QR (in *qr_rank_h),
+  // statistics, X sized for Y = R^H; returns the arguments of building Y
+  RhArgs qr_finish_host(int m, int n, int r, float ms) {
     require(r >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
     qr_ms += ms;
     qr_count += 1;
     tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = ms;
     X.ensure((size_t)n * r, svd_stream);
+    qr_m = m; qr_n = n; qr_r = r;
+    RhArgs ra;
+    ra.A = Q.p; ra.lda = m; ra.n = n; ra.pad = 0;
+    ra.piv = qr_piv; ra.beta = qr_beta; ra.rank = qr_rank_d; ra.Y = X.p;
+    return ra;
+  }
+
+  int qr_finish(int m, int n, float ms) {
+    const int r = *qr_rank_h;
+    qr_finish_host(m, n, r, ms);
     build_rh_kernel<<<grid_for((long)n * r), 256, 0, svd_stream>>>(Q.p, m, n, qr_piv,
                                                                    qr_beta, qr_rank_d, X.p);
-    qr_m = m; qr_n = n; qr_r = r;
     return r;
+  }
+
+  SvdPostArgs post_args(int m, int n) const {
+    SvdPostArgs pa;
+    pa.X = X.p; pa.m = m; pa.n = n;
+    pa.sigma = sigma; pa.perm = perm; pa.sorted = sorted; pa.info = info_d;
+    pa.epsrel = epsrel; pa.max_rank = chi_cap; pa.pad = 0;
+    return pa;
   }
 
   int batched_svd(bool qr, int m, int n);
@@ -620,6 +637,8 @@ struct Batcher {
     (void)hipStreamSynchronize(stream);
     (void)hipFree(stage_d);
     (void)hipHostFree(stage_h);
+    (void)hipHostFree(ranks_h); (void)hipHostFree(infos_h);
+    (void)hipFree(ranks_d); (void)hipFree(infos_d);
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     (void)hipStreamDestroy(stream);
@@ -699,32 +718,35 @@ struct Batcher {
     return out;
   }
 
-  // copy the per-request arguments, starts and the workgroup map to the device
-  template <class Args>
-  void upload(const std::vector<Args>& args, const std::vector<int>& grids, Args** dargs,
-              int** dstarts, int** dmap, int* total) {
-    const size_t na = args.size();
-    int t = 0;
-    for (int g : grids) t += g;
-    *total = t;
-    const size_t b_args = na * sizeof(Args);
-    const size_t o_starts = (b_args + 255) / 256 * 256;
-    const size_t o_map = o_starts + ((na + 1) * sizeof(int) + 255) / 256 * 256;
-    const size_t bytes = o_map + (size_t)t * sizeof(int);
-    ensure_stage(bytes);
-    HIP_CHECK(hipStreamSynchronize(stream));   // the staging buffer is free again
-    std::memcpy(stage_h, args.data(), b_args);
-    int* st = reinterpret_cast<int*>(stage_h + o_starts);
-    int* mp = reinterpret_cast<int*>(stage_h + o_map);
-    st[0] = 0;
-    for (size_t p = 0; p < na; ++p) {
-      st[p + 1] = st[p] + grids[p];
-      for (int b = st[p]; b < st[p + 1]; ++b) mp[b] = (int)p;
+  // host image of several argument arrays, copied to the device at once
+  struct Pack {
+    std::vector<char> h;
+    size_t add(const void* p, size_t bytes) {
+      const size_t off = (h.size() + 255) / 256 * 256;
+      h.resize(off + bytes);
+      if (bytes) std::memcpy(h.data() + off, p, bytes);
+      return off;
     }
-    HIP_CHECK(hipMemcpyAsync(stage_d, stage_h, bytes, hipMemcpyHostToDevice, stream));
-    *dargs = reinterpret_cast<Args*>(stage_d);
-    *dstarts = reinterpret_cast<int*>(stage_d + o_starts);
-    *dmap = reinterpret_cast<int*>(stage_d + o_map);
+  };
+
+  // copy a pack to the device (after the previous one has been consumed);
+  // returns the device base address
+  char* upload(const Pack& pk) {
+    ensure_stage(pk.h.size());
+    HIP_CHECK(hipStreamSynchronize(stream));
+    std::memcpy(stage_h, pk.h.data(), pk.h.size());
+    HIP_CHECK(hipMemcpyAsync(stage_d, stage_h, pk.h.size(), hipMemcpyHostToDevice, stream));
+    return stage_d;
+  }
+
+  // starts and workgroup->problem map of a batched cooperative launch
+  static void layout(const std::vector<int>& grids, std::vector<int>* starts,
+                     std::vector<int>* map) {
+    starts->assign(grids.size() + 1, 0);
+    for (size_t p = 0; p < grids.size(); ++p) (*starts)[p + 1] = (*starts)[p] + grids[p];
+    map->assign((*starts)[grids.size()], 0);
+    for (size_t p = 0; p < grids.size(); ++p)
+      for (int b = (*starts)[p]; b < (*starts)[p + 1]; ++b) (*map)[b] = (int)p;
   }
 
   // workgroups per request: its minimum, plus a share of what is left of
@@ -752,80 +774,114 @@ struct Batcher {
     r->err = std::current_exception();
   }
 
-  void execute(std::vector<SvdRequest*>& batch) {
-    for (auto* r : batch) {
-      HIP_CHECK(hipStreamWaitEvent(stream, r->ws->ev_ready, 0));
-      r->ws->svd_stream = stream;
-    }
-    // ---- pivoted QRs ----
-    std::vector<SvdRequest*> qrs;
-    for (auto* r : batch)
-      if (r->qr) qrs.push_back(r);
-    if (!qrs.empty()) {
-      int max_m = 0;
-      for (auto* r : qrs) max_m = std::max(max_m, r->m);
-      const size_t lds = (size_t)max_m * sizeof(double2);
-      int per_cu = 0;
-      HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(qrcp_batch_kernel), 256, lds));
-      const int cap = std::min(per_cu * num_cus, 1024);
-      require(cap > 0, PTE_ERR_DEVICE, "batched QR kernel cannot be resident");
-      auto ming = [](SvdRequest* r) { return Workspace::qr_min_grid(r->n); };
-      std::vector<SvdRequest*> ok;
-      for (auto& ch : chunks(qrs, cap, ming)) {
-        std::vector<int> nat, mn;
-        for (size_t i = ch.first; i < ch.second; ++i) {
-          nat.push_back(qrs[i]->ws->qr_natural_grid(qrs[i]->n));
-          mn.push_back(Workspace::qr_min_grid(qrs[i]->n));
-        }
-        const std::vector<int> alloc = allocate(nat, mn, cap);
-        std::vector<QrcpArgs> args;
-        std::vector<int> grids;
-        std::vector<SvdRequest*> in;
-        for (size_t i = ch.first; i < ch.second; ++i) {
-          SvdRequest* r = qrs[i];
-          try {
-            const int g = alloc[i - ch.first];
-            args.push_back(r->ws->qr_prepare(r->m, r->n, g));
-            grids.push_back(g);
-            in.push_back(r);
-          } catch (...) {
-            fail(r);
-          }
-        }
-        if (in.empty()) continue;
-        QrcpArgs* da; int* ds; int* dm; int total = 0;
-        upload(args, grids, &da, &ds, &dm, &total);
-        require(total <= cap, PTE_ERR_CAPACITY, "internal: batched QR grid too large");
-        void* kargs[] = {&da, &ds, &dm};
-        HIP_CHECK(hipEventRecord(ev0, stream));
-        HIP_CHECK(hipLaunchCooperativeKernel(
-            reinterpret_cast<const void*>(qrcp_batch_kernel), dim3(total), dim3(256), kargs,
-            lds, stream));
-        HIP_CHECK(hipEventRecord(ev1, stream));
-        for (auto* r : in)
-          HIP_CHECK(hipMemcpyAsync(r->ws->qr_rank_h, r->ws->qr_rank_d, sizeof(int),
-                                   hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        float ms = 0.f;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-        for (auto* r : in) {
-          try {
-            const int rk = r->ws->qr_finish(r->m, r->n, ms);
-            r->jm = r->n;
-            r->jn = rk;
-          } catch (...) {
-            fail(r);
-          }
+  // pinned host arrays for the gathered QR ranks and SVD summaries
+  int* ranks_h = nullptr;
+  SvdInfo* infos_h = nullptr;
+  int* ranks_d = nullptr;
+  SvdInfo* infos_d = nullptr;
+  size_t gather_cap = 0;
+
+  void ensure_gather(size_t n) {
+    if (n <= gather_cap) return;
+    HIP_CHECK(hipStreamSynchronize(stream));
+    (void)hipHostFree(ranks_h); (void)hipHostFree(infos_h);
+    (void)hipFree(ranks_d); (void)hipFree(infos_d);
+    const size_t c = std::max<size_t>(n, 256);
+    HIP_CHECK(hipHostMalloc((void**)&ranks_h, c * sizeof(int), hipHostMallocDefault));
+    HIP_CHECK(hipHostMalloc((void**)&infos_h, c * sizeof(SvdInfo), hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void**)&ranks_d, c * sizeof(int)));
+    HIP_CHECK(hipMalloc((void**)&infos_d, c * sizeof(SvdInfo)));
+    gather_cap = c;
+  }
+
+  void run_qr(std::vector<SvdRequest*>& qrs) {
+    int max_m = 0;
+    for (auto* r : qrs) max_m = std::max(max_m, r->m);
+    const size_t lds = (size_t)max_m * sizeof(double2);
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(qrcp_batch_kernel), 256, lds));
+    const int cap = std::min(per_cu * num_cus, 1024);
+    require(cap > 0, PTE_ERR_DEVICE, "batched QR kernel cannot be resident");
+    auto ming = [](SvdRequest* r) { return Workspace::qr_min_grid(r->n); };
+    for (auto& ch : chunks(qrs, cap, ming)) {
+      std::vector<int> nat, mn;
+      for (size_t i = ch.first; i < ch.second; ++i) {
+        nat.push_back(qrs[i]->ws->qr_natural_grid(qrs[i]->n));
+        mn.push_back(Workspace::qr_min_grid(qrs[i]->n));
+      }
+      const std::vector<int> alloc = allocate(nat, mn, cap);
+      std::vector<QrcpArgs> args;
+      std::vector<ZeroItem> zeros;
+      std::vector<int> grids;
+      std::vector<SvdRequest*> in;
+      for (size_t i = ch.first; i < ch.second; ++i) {
+        SvdRequest* r = qrs[i];
+        try {
+          const int g = alloc[i - ch.first];
+          args.push_back(r->ws->qr_prepare(r->m, r->n, g));
+          zeros.push_back(ZeroItem{r->ws->bar, (int)kBarrierWords, 0});
+          grids.push_back(g);
+          in.push_back(r);
+        } catch (...) {
+          fail(r);
         }
       }
+      if (in.empty()) continue;
+      std::vector<int> starts, map;
+      layout(grids, &starts, &map);
+      const int total = starts.back();
+      require(total <= cap, PTE_ERR_CAPACITY, "internal: batched QR grid too large");
+      ensure_gather(in.size());
+      Pack pk;
+      const size_t oa = pk.add(args.data(), args.size() * sizeof(QrcpArgs));
+      const size_t os = pk.add(starts.data(), starts.size() * sizeof(int));
+      const size_t om = pk.add(map.data(), map.size() * sizeof(int));
+      const size_t oz = pk.add(zeros.data(), zeros.size() * sizeof(ZeroItem));
+      char* base = upload(pk);
+      const QrcpArgs* da = reinterpret_cast<const QrcpArgs*>(base + oa);
+      const int* ds = reinterpret_cast<const int*>(base + os);
+      const int* dm = reinterpret_cast<const int*>(base + om);
+      zero_items_kernel<<<(int)zeros.size(), 256, 0, stream>>>(
+          reinterpret_cast<const ZeroItem*>(base + oz));
+      void* kargs[] = {&da, &ds, &dm};
+      HIP_CHECK(hipEventRecord(ev0, stream));
+      HIP_CHECK(hipLaunchCooperativeKernel(
+          reinterpret_cast<const void*>(qrcp_batch_kernel), dim3(total), dim3(256), kargs,
+          lds, stream));
+      HIP_CHECK(hipEventRecord(ev1, stream));
+      gather_qr_rank_kernel<<<1, 256, 0, stream>>>(da, (int)in.size(), ranks_d);
+      HIP_CHECK(hipMemcpyAsync(ranks_h, ranks_d, in.size() * sizeof(int),
+                               hipMemcpyDeviceToHost, stream));
+      HIP_CHECK(hipStreamSynchronize(stream));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      std::vector<RhArgs> rh;
+      long max_elems = 0;
+      for (size_t i = 0; i < in.size(); ++i) {
+        SvdRequest* r = in[i];
+        try {
+          const int rk = ranks_h[i];
+          rh.push_back(r->ws->qr_finish_host(r->m, r->n, rk, ms));
+          r->jm = r->n;
+          r->jn = rk;
+          max_elems = std::max(max_elems, (long)r->n * rk);
+        } catch (...) {
+          fail(r);
+        }
+      }
+      if (rh.empty()) continue;
+      Pack pr;
+      const size_t orh = pr.add(rh.data(), rh.size() * sizeof(RhArgs));
+      char* b2 = upload(pr);
+      const int gx = std::max(1, std::min<int>((int)((max_elems + 255) / 256), 1024));
+      build_rh_batch_kernel<<<dim3(gx, (unsigned)rh.size()), 256, 0, stream>>>(
+          reinterpret_cast<const RhArgs*>(b2 + orh));
+      HIP_CHECK(hipGetLastError());
     }
-    for (auto* r : batch)
-      if (!r->qr) { r->jm = r->m; r->jn = r->n; }
-    // ---- Jacobi SVDs ----
-    std::vector<SvdRequest*> jr;
-    for (auto* r : batch)
-      if (!r->failed) jr.push_back(r);
+  }
+
+  void run_jacobi(std::vector<SvdRequest*>& jr) {
     const int cap = max_jacobi_grid;
     auto ming = [](SvdRequest* r) { return Workspace::jacobi_blocks(r->jn) / 2; };
     for (auto& ch : chunks(jr, cap, ming)) {
@@ -836,42 +892,90 @@ struct Batcher {
       }
       const std::vector<int> alloc = allocate(nat, mn, cap);
       std::vector<JacobiArgs> args;
+      std::vector<SvdPostArgs> post;
+      std::vector<ZeroItem> zeros;
       std::vector<int> grids;
       std::vector<SvdRequest*> in;
+      int max_n = 1;
       for (size_t i = ch.first; i < ch.second; ++i) {
         SvdRequest* r = jr[i];
         try {
           int g = 0;
           args.push_back(r->ws->jacobi_prepare(r->jm, r->jn, alloc[i - ch.first], &g));
-          r->ws->jacobi_reset();
+          post.push_back(r->ws->post_args(r->jm, r->jn));
+          Workspace* w = r->ws;
+          zeros.push_back(ZeroItem{w->bar, (int)kBarrierWords, 0});
+          zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->flags), w->max_sweeps, 0});
+          zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->fro2), 2, 0});
           grids.push_back(g);
           in.push_back(r);
+          max_n = std::max(max_n, r->jn);
         } catch (...) {
           fail(r);
         }
       }
       if (in.empty()) continue;
-      JacobiArgs* da; int* ds; int* dm; int total = 0;
-      upload(args, grids, &da, &ds, &dm, &total);
+      std::vector<int> starts, map;
+      layout(grids, &starts, &map);
+      const int total = starts.back();
       require(total <= cap, PTE_ERR_CAPACITY, "internal: batched Jacobi grid too large");
+      ensure_gather(in.size());
+      Pack pk;
+      const size_t oa = pk.add(args.data(), args.size() * sizeof(JacobiArgs));
+      const size_t os = pk.add(starts.data(), starts.size() * sizeof(int));
+      const size_t om = pk.add(map.data(), map.size() * sizeof(int));
+      const size_t oz = pk.add(zeros.data(), zeros.size() * sizeof(ZeroItem));
+      const size_t op = pk.add(post.data(), post.size() * sizeof(SvdPostArgs));
+      char* base = upload(pk);
+      const JacobiArgs* da = reinterpret_cast<const JacobiArgs*>(base + oa);
+      const int* ds = reinterpret_cast<const int*>(base + os);
+      const int* dm = reinterpret_cast<const int*>(base + om);
+      const SvdPostArgs* dp = reinterpret_cast<const SvdPostArgs*>(base + op);
+      zero_items_kernel<<<(int)zeros.size(), 256, 0, stream>>>(
+          reinterpret_cast<const ZeroItem*>(base + oz));
       void* kargs[] = {&da, &ds, &dm};
       HIP_CHECK(hipEventRecord(ev0, stream));
       HIP_CHECK(hipLaunchCooperativeKernel(
           reinterpret_cast<const void*>(jacobi_batch_kernel), dim3(total), dim3(256), kargs,
           0, stream));
       HIP_CHECK(hipEventRecord(ev1, stream));
-      for (auto* r : in) r->ws->jacobi_finish(r->jm, r->jn);
+      const unsigned np = (unsigned)in.size();
+      column_norm_batch_kernel<<<dim3(max_n, np), 256, 0, stream>>>(dp);
+      rank_sort_batch_kernel<<<dim3((max_n + 255) / 256, np), 256, 0, stream>>>(dp);
+      truncate_batch_kernel<<<np, 1024, 0, stream>>>(dp, infos_d);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(infos_h, infos_d, in.size() * sizeof(SvdInfo),
+                               hipMemcpyDeviceToHost, stream));
       HIP_CHECK(hipStreamSynchronize(stream));
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-      for (auto* r : in) {
+      for (size_t i = 0; i < in.size(); ++i) {
+        SvdRequest* r = in[i];
         try {
+          *r->ws->info_h = infos_h[i];
           r->k = r->ws->jacobi_check(r->jm, r->jn, ms);
         } catch (...) {
           fail(r);
         }
       }
     }
+  }
+
+  void execute(std::vector<SvdRequest*>& batch) {
+    for (auto* r : batch) {
+      HIP_CHECK(hipStreamWaitEvent(stream, r->ws->ev_ready, 0));
+      r->ws->svd_stream = stream;
+    }
+    std::vector<SvdRequest*> qrs;
+    for (auto* r : batch) {
+      if (r->qr) qrs.push_back(r);
+      else { r->jm = r->m; r->jn = r->n; }
+    }
+    if (!qrs.empty()) run_qr(qrs);
+    std::vector<SvdRequest*> jr;
+    for (auto* r : batch)
+      if (!r->failed) jr.push_back(r);
+    if (!jr.empty()) run_jacobi(jr);
     for (auto* r : batch) r->ws->svd_stream = r->ws->stream;
   }
 };
@@ -1945,9 +2049,10 @@ int pte_pt_batch_create(pte_pt_engine* const* engines, int count, pte_pt_batch**
   });
 }
 
-int pte_pt_batch_step(pte_pt_batch* b, int* more) {
+int pte_pt_batch_run(pte_pt_batch* b, int max_steps, int* more) {
   return guarded([&] {
     require(b != nullptr && more != nullptr, PTE_ERR_ARG, "null argument");
+    require(max_steps >= 1, PTE_ERR_ARG, "max_steps must be positive");
     std::vector<PtEngine*> active;
     for (auto* e : b->engines) {
       require(e->impl.step >= 1, PTE_ERR_STATE, "engine in batch not initialised");
@@ -1963,7 +2068,8 @@ int pte_pt_batch_step(pte_pt_batch* b, int* more) {
           PtEngine* eng = active[i];
           eng->ws.batcher = &b->batcher;
           try {
-            eng->compute_step(nullptr, 0, 0);
+            for (int t = 0; t < max_steps && eng->step < eng->cfg.num_steps; ++t)
+              eng->compute_step(nullptr, 0, 0);
           } catch (...) {
             errs[i] = std::current_exception();
           }

@@ -98,11 +98,36 @@ struct ApplyQArgs {
 
 constexpr int kQrcpMaxLocal = 64;         // columns per workgroup in qrcp_kernel
 
+// One SVD's singular values, order and cut (batched post-processing).
+struct SvdPostArgs {
+  const double2* X; int m, n;               // orthogonalised X (ld = m)
+  double* sigma; int* perm; double* sorted;
+  SvdInfo* info;
+  double epsrel; int max_rank; int pad;
+};
+
+// Y = R^H of one pivoted QR (batched).
+struct RhArgs {
+  const double2* A; long lda; int n; int pad;
+  const int* piv; const double* beta; const int* rank;
+  double2* Y;
+};
+
+struct ZeroItem {
+  unsigned* p; int words; int pad;
+};
+
 __global__ void qrcp_kernel(QrcpArgs a);
 __global__ void qrcp_batch_kernel(const QrcpArgs* probs, const int* starts,
                                   const int* block_prob);
 __global__ void build_rh_kernel(const double2* A, long lda, int n, const int* piv,
                                 const double* beta, const int* rank, double2* Y);
+__global__ void build_rh_batch_kernel(const RhArgs* pa);
+__global__ void column_norm_batch_kernel(const SvdPostArgs* pa);
+__global__ void rank_sort_batch_kernel(const SvdPostArgs* pa);
+__global__ void truncate_batch_kernel(const SvdPostArgs* pa, SvdInfo* out);
+__global__ void zero_items_kernel(const ZeroItem* items);
+__global__ void gather_qr_rank_kernel(const QrcpArgs* probs, int n, int* out);
 __global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);
 __global__ void cgemm_mfma_kernel(GemmArgs a);

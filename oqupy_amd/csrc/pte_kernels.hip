@@ -613,10 +613,9 @@ jacobi_batch_kernel(const JacobiArgs* probs, const int* starts, const int* block
 }
 
 // sigma_j = || X[:, j] ||  (one workgroup per column)
-__global__ void __launch_bounds__(256)
-column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma) {
+__device__ __forceinline__ void column_norm_body(const double2* X, int m, int n, long ldx,
+                                                 double* sigma, int j) {
   __shared__ double red[256];
-  const int j = blockIdx.x;
   if (j >= n) return;
   double s = 0.0;
   for (int i = threadIdx.x; i < m; i += 256) s += cabs2(X[i + (long)j * ldx]);
@@ -629,10 +628,14 @@ column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma) {
   if (threadIdx.x == 0) sigma[j] = sqrt(red[0]);
 }
 
-// descending order: rank_j = #{i : s_i > s_j} + #{i < j : s_i == s_j}
 __global__ void __launch_bounds__(256)
-rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
+column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma) {
+  column_norm_body(X, m, n, ldx, sigma, blockIdx.x);
+}
+
+// descending order, ties by index: perm[rank] = j, sorted[rank] = sigma[j]
+__device__ __forceinline__ void rank_sort_body(const double* sigma, int n, int* perm,
+                                               double* sorted, int j) {
   if (j >= n) return;
   const double sj = sigma[j];
   int rank = 0;
@@ -644,10 +647,13 @@ rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted) {
   sorted[rank] = sj;
 }
 
-// k = #{ j : sqrt(sum_{i>=j} s_i^2) > epsrel * s_0 }  (single workgroup)
-__global__ void __launch_bounds__(1024)
-truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
-                int max_rank) {
+__global__ void __launch_bounds__(256)
+rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted) {
+  rank_sort_body(sigma, n, perm, sorted, blockIdx.x * 256 + threadIdx.x);
+}
+
+__device__ __forceinline__ void truncate_body(const double* sorted, int n, double epsrel,
+                                              SvdInfo* info, int max_rank, SvdInfo* copy) {
   __shared__ double part[1024];
   __shared__ int cnt[1024];
   const int tid = threadIdx.x;
@@ -690,8 +696,51 @@ truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
     double d = 0.0;
     for (int i = n - 1; i >= k; --i) d += sorted[i] * sorted[i];
     info->discarded = sqrt(d);
+    if (copy) *copy = *info;
   }
 }
+
+__global__ void __launch_bounds__(1024)
+truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
+                int max_rank) {
+  truncate_body(sorted, n, epsrel, info, max_rank, nullptr);
+}
+
+// ---- the same post-processing for many SVDs at once (problem = blockIdx.y,
+// or blockIdx.x for the truncation); see SvdPostArgs ----
+__global__ void __launch_bounds__(256)
+column_norm_batch_kernel(const SvdPostArgs* pa) {
+  const SvdPostArgs a = pa[blockIdx.y];
+  column_norm_body(a.X, a.m, a.n, a.m, a.sigma, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256)
+rank_sort_batch_kernel(const SvdPostArgs* pa) {
+  const SvdPostArgs a = pa[blockIdx.y];
+  rank_sort_body(a.sigma, a.n, a.perm, a.sorted, blockIdx.x * 256 + threadIdx.x);
+}
+
+// also gathers every SvdInfo into `out` for a single copy to the host
+__global__ void __launch_bounds__(1024)
+truncate_batch_kernel(const SvdPostArgs* pa, SvdInfo* out) {
+  const SvdPostArgs a = pa[blockIdx.x];
+  truncate_body(a.sorted, a.n, a.epsrel, a.info, a.max_rank, out + blockIdx.x);
+}
+
+// zero a list of device words (barrier counters, sweep flags): item = blockIdx.x
+__global__ void __launch_bounds__(256)
+zero_items_kernel(const ZeroItem* items) {
+  const ZeroItem it = items[blockIdx.x];
+  for (int i = threadIdx.x; i < it.words; i += 256) it.p[i] = 0u;
+}
+
+// out[i] = *probs[i].rank (pivoted-QR ranks for a single copy to the host)
+__global__ void __launch_bounds__(256)
+gather_qr_rank_kernel(const QrcpArgs* probs, int n, int* out) {
+  for (int i = threadIdx.x; i < n; i += 256) out[i] = *probs[i].rank;
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Emission of the truncated factors. After the Jacobi on X = M^H (m' x n'):
@@ -944,12 +993,13 @@ qrcp_batch_kernel(const QrcpArgs* probs, const int* starts, const int* block_pro
 
 // Y = R^H (n x r, column-major, ld = n) in the pivoted column order:
 // Y[s + t*n] = conj(R[t][piv[s]])
-__global__ void __launch_bounds__(256)
-build_rh_kernel(const double2* A, long lda, int n, const int* piv, const double* beta,
-                const int* rank, double2* Y) {
+__device__ __forceinline__ void build_rh_body(const double2* A, long lda, int n,
+                                              const int* piv, const double* beta,
+                                              const int* rank, double2* Y, int bid,
+                                              int nblocks) {
   const int r = *rank;
   const long total = (long)n * r;
-  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+  for (long e = bid * 256L + threadIdx.x; e < total; e += (long)nblocks * 256) {
     const int s = e % n;
     const int t = e / n;
     double2 v = make_double2(0.0, 0.0);
@@ -957,6 +1007,18 @@ build_rh_kernel(const double2* A, long lda, int n, const int* piv, const double*
     else if (t == s) v = make_double2(beta[s], 0.0);
     Y[e] = v;
   }
+}
+
+__global__ void __launch_bounds__(256)
+build_rh_kernel(const double2* A, long lda, int n, const int* piv, const double* beta,
+                const int* rank, double2* Y) {
+  build_rh_body(A, lda, n, piv, beta, rank, Y, blockIdx.x, gridDim.x);
+}
+
+__global__ void __launch_bounds__(256)
+build_rh_batch_kernel(const RhArgs* pa) {
+  const RhArgs a = pa[blockIdx.y];
+  build_rh_body(a.A, a.lda, a.n, a.piv, a.beta, a.rank, a.Y, blockIdx.x, gridDim.x);
 }
 
 // Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR: one

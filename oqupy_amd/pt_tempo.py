@@ -222,8 +222,9 @@ def pt_tempo_compute_batch(baths: List[Bath],
     eng.check(lib.pte_pt_batch_create(handles, count, ctypes.byref(batch)))
     try:
         more = np.ones(count, dtype=np.int32)
+        steps = max(p.num_steps for p in ptts)
         while more.any():
-            eng.check(lib.pte_pt_batch_step(batch, eng.iptr(more)))
+            eng.check(lib.pte_pt_batch_run(batch, steps, eng.iptr(more)))
     finally:
         lib.pte_pt_batch_destroy(batch)
     for b in backends:
