@@ -113,6 +113,117 @@ def cpu_baseline(warmup, steps, table, budget_s):
     return done / (time.perf_counter() - t0), done
 
 
+C4_SEED, C4_N, C4_DT, C4_EPSREL = 0, 500, 0.05, 1e-7
+
+
+def c4_instance(i):
+    """Instance i of the seeded sweep of spin-boson baths: alpha ~ U[0.05, 0.8],
+    w_c ~ U[1, 10], T ~ U[0, 1], dkmax ~ U{50..300}, dt 0.05, epsrel 1e-7."""
+    import numpy as np
+    rng = np.random.default_rng(C4_SEED)
+    alphas = rng.uniform(0.05, 0.8, 256)
+    wcs = rng.uniform(1.0, 10.0, 256)
+    temps = rng.uniform(0.0, 1.0, 256)
+    dkmaxs = rng.integers(50, 301, 256)
+    return float(alphas[i]), float(wcs[i]), float(temps[i]), int(dkmaxs[i])
+
+
+def c4_table(i):
+    from oqupy_amd import operators as op
+    from oqupy_amd.bath import Bath
+    from oqupy_amd.bath_correlations import PowerLawSD
+    from oqupy_amd.tempo import TempoParameters, influence_matrix
+    a, wc, t, k = c4_instance(i)
+    corr = PowerLawSD(alpha=a, zeta=1.0, cutoff=wc, cutoff_type="exponential",
+                      temperature=t)
+    bath = Bath(0.5 * op.sigma("z"), corr)
+    params = TempoParameters(dt=C4_DT, dkmax=k, epsrel=C4_EPSREL)
+    return {dk: influence_matrix(dk, params, bath.correlations,
+                                 bath.coupling_acomm, bath.coupling_comm)
+            for dk in range(k + 1)}
+
+
+def c4_engines(count, tables):
+    import oqupy_amd as oqupy
+    from oqupy_amd import operators as op
+    out = []
+    for i in range(count):
+        a, wc, t, k = c4_instance(i)
+        corr = oqupy.PowerLawSD(alpha=a, zeta=1.0, cutoff=wc,
+                                cutoff_type="exponential", temperature=t)
+        bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
+        params = oqupy.TempoParameters(dt=C4_DT, dkmax=k, epsrel=C4_EPSREL)
+        ptt = oqupy.PtTempo(bath, 0.0, C4_N * C4_DT + 1e-9, params,
+                            backend_config={"chi_cap": 8192})
+        tab = tables[i]
+        ptt.backend._influence = lambda dk, tab=tab: tab.get(dk)
+        ptt.backend.initialize()
+        out.append(ptt.backend)
+    return out
+
+
+def batch_main(args):
+    """Batched form: the first --batch instances of the seeded sweep, one
+    warm-up step each, then --steps steps each, all engines stepped together
+    through batched SVD launches; the same instances stepped one by one on
+    the GPU are timed next to it."""
+    import ctypes
+    from concurrent.futures import ProcessPoolExecutor
+    import numpy as np
+    import torch
+    from oqupy_amd.backends import engine as eng
+    count = args.batch
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+        tables = list(ex.map(c4_table, range(count)))
+    precompute_s = time.perf_counter() - t0
+    lib = eng.load_library()
+    backends = c4_engines(count, tables)
+    handles = (ctypes.c_void_p * count)(*[b.handle.value for b in backends])
+    batch = ctypes.c_void_p()
+    eng.check(lib.pte_pt_batch_create(handles, count, ctypes.byref(batch)))
+    more = np.ones(count, dtype=np.int32)
+    eng.check(lib.pte_pt_batch_run(batch, 1, eng.iptr(more)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.check(lib.pte_pt_batch_run(batch, args.steps, eng.iptr(more)))
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    lib.pte_pt_batch_destroy(batch)
+    for b in backends:
+        b.sync_step()
+    bonds = [max(b.bond_dimensions()) for b in backends]
+    del backends
+    single = c4_engines(count, tables)
+    for b in single:
+        b.compute_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for b in single:
+            b.compute_step()
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    value = count * args.steps / t_batch
+    print(json.dumps({
+        "metric": "batched PT-TEMPO instance-steps/s (sum over instances)",
+        "value": round(value, 4), "unit": "instance-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * t_batch / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"first {count} instances of the seeded sweep "
+                               f"(alpha~U[0.05,0.8], wc~U[1,10], T~U[0,1], "
+                               f"dkmax~U{{50..300}}, dt=0.05, epsrel=1e-7, "
+                               f"N=500, seed 0), steps 2..{1 + args.steps}",
+                   "max_bond_in_window": int(max(bonds)),
+                   "one_by_one_instance_steps_per_s": round(
+                       count * args.steps / t_single, 4),
+                   "batch_speedup": round(t_single / t_batch, 3),
+                   "influence_precompute_s": round(precompute_s, 2)},
+        "roofline": None, "cpu_baseline": None}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +232,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of timed oracle work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="time the batched form on this many instances of the "
+                         "seeded sweep of baths instead of the headline run")
     ap.add_argument("--precision", choices=("c128", "c64"), default="c128",
                     help="c64: contractions in complex64 on the matrix cores "
                          "(FP32 accumulation), SVD and cut in FP64")
@@ -130,6 +244,11 @@ def main():
                           "single PT-TEMPO instance is a sequential chain of "
                           "SVDs and is not sharded across GPUs"}))
         sys.exit(2)
+    if args.batch > 0:
+        if args.batch > 256:
+            raise SystemExit("--batch takes at most 256 instances")
+        batch_main(args)
+        return
 
     bath, params, np = build_model()
     t0 = time.perf_counter()
