@@ -95,6 +95,13 @@ __device__ __forceinline__ int ld_agent(const int* p) {
 __device__ __forceinline__ void st_agent(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
+  return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ double2 ld_agent(const double2* p) {
   return make_double2(ld_agent(&p->x), ld_agent(&p->y));
 }
@@ -109,6 +116,10 @@ __device__ __forceinline__ void st_agent(double2* p, double2 v) {
 // wavefront first waits for its outstanding memory operations (s_waitcnt 0),
 // so its agent-scope stores are performed before its workgroup arrives.
 constexpr unsigned kBarrierWords = 64;
+// words of the synchronisation buffer: the grid counter, then (Jacobi) a pair
+// barrier and a pair completion counter per block pair and a version per block
+constexpr unsigned kSyncMaxBlocks = 8192;
+constexpr unsigned kSyncWords = kBarrierWords + 2 * kSyncMaxBlocks;
 
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks,
                                              unsigned& epoch) {

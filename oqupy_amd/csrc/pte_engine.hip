@@ -207,6 +207,7 @@ struct Workspace {
   SvdInfo* info_h = nullptr;
   int max_sweeps = 80;
   int inner_sweeps = 1;      // tuning: PTE_JACOBI_INNER_SWEEPS
+  int jacobi_sync_words = kBarrierWords;   // words of `bar` the last prepared Jacobi uses
   int qr_blocks = 256;       // tuning: PTE_QR_BLOCKS (workgroups of the pivoted QR)
   int slice_rows = 64;       // tuning: PTE_JACOBI_SLICE_ROWS (min rows per slice)
   int max_coop_blocks = 0;
@@ -241,7 +242,7 @@ struct Workspace {
       uint64_t thr = ~0ull;
       (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
-    HIP_CHECK(hipMalloc((void**)&bar, sizeof(unsigned) * kBarrierWords));
+    HIP_CHECK(hipMalloc((void**)&bar, sizeof(unsigned) * kSyncWords));
     HIP_CHECK(hipMalloc((void**)&flags, sizeof(int) * max_sweeps));
     HIP_CHECK(hipMalloc((void**)&fro2, sizeof(double)));
     HIP_CHECK(hipMalloc((void**)&info_d, sizeof(SvdInfo)));
@@ -319,6 +320,8 @@ struct Workspace {
     ensure_sv((size_t)n);
     const int nb = jacobi_blocks(n);
     const int npairs = nb / 2;
+    require(nb <= (int)kSyncMaxBlocks, PTE_ERR_CAPACITY,
+            "SVD of " + std::to_string(n) + " columns exceeds the Jacobi's block limit");
     require(npairs <= max_grid, PTE_ERR_CAPACITY,
             "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
     // row slices per block pair: each slice keeps at least slice_rows rows of
@@ -327,7 +330,8 @@ struct Workspace {
                                         (m + slice_rows - 1) / slice_rows, 16}));
     auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
     const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
-    if (R > 1) gpart.ensure((size_t)npairs * R * 1024, svd_stream);
+    if (R > 1) gpart.ensure((size_t)2 * npairs * R * 1024, svd_stream);
+    jacobi_sync_words = kBarrierWords + 2 * nb;
     JacobiArgs ja;
     ja.X = X.p; ja.m = m; ja.n = n; ja.ldx = m;
     ja.V = V.p; ja.ldv = n;
@@ -359,7 +363,7 @@ struct Workspace {
   }
 
   void jacobi_reset() {
-    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
+    HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_sync_words, svd_stream));
     HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, svd_stream));
     HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), svd_stream));
   }
@@ -904,7 +908,7 @@ struct Batcher {
           args.push_back(r->ws->jacobi_prepare(r->jm, r->jn, alloc[i - ch.first], &g));
           post.push_back(r->ws->post_args(r->jm, r->jn));
           Workspace* w = r->ws;
-          zeros.push_back(ZeroItem{w->bar, (int)kBarrierWords, 0});
+          zeros.push_back(ZeroItem{w->bar, w->jacobi_sync_words, 0});
           zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->flags), w->max_sweeps, 0});
           zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->fro2), 2, 0});
           grids.push_back(g);

@@ -298,7 +298,12 @@ mpo_apply_right_kernel(MpoApplyArgs a) {
 // two-sided Jacobi in LDS, at most inner_sweeps sweeps, identically in every
 // workgroup of the pair) and the resulting unitary applied to the slice's rows
 // of X and V. A sweep is nb-1 rounds; the iteration stops after a sweep in
-// which no pair needed a rotation. A pair needs one when some
+// which no pair needed a rotation. Rounds are not separated by grid barriers:
+// every block carries a version (rounds completed on it), a pair starts a
+// round once both of its blocks have reached it and, when all of its slices
+// are done, releases them for the next round; the slices of a pair exchange
+// partial Grams through a pair-local barrier (double-buffered by round
+// parity); a grid barrier closes each sweep. A pair needs one when some
 // |g_ij| > tol*sqrt(g_ii g_jj) and |g_ij| > abs_tol*sqrt(max(g_ii, g_jj)).
 // ---------------------------------------------------------------------------
 constexpr int JB = 16;          // block width
@@ -348,6 +353,32 @@ __device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 
   return true;
 }
 
+// One 32-row chunk of the pair's 32 columns (rows r0.., below hi), four
+// entries per thread (entry e = tid + 256 q: row e % 32, column e / 32), read
+// with agent-scope loads; PF chunks are issued before any is consumed so
+// that their latencies overlap.
+constexpr int PF = 4;
+
+__device__ __forceinline__ void load_chunk(double2 (&out)[4], const double2* base, long ld,
+                                           int r0, int hi, const int* cols) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + JT * q;
+    const int rr = e % JR, cc = e / JR;
+    const int gr = r0 + rr, gc = cols[cc];
+    out[q] = (gr < hi && gc >= 0) ? ld_agent(base + (gr + (long)gc * ld))
+                                  : make_double2(0.0, 0.0);
+  }
+}
+
+__device__ __forceinline__ void put_chunk(double2 (*Xs)[JP + 1], const double2 (&in)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + JT * q;
+    Xs[e % JR][e / JR] = in[q];
+  }
+}
+
 __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
                                             const unsigned nblk) {
   __shared__ double2 Xs[JR][JP + 1];
@@ -389,6 +420,11 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
   const double abs_tol = a.abs_tol_scale * fro;
   const double tol2 = a.tol * a.tol, abs_tol2 = abs_tol * abs_tol;
   const int ti = tid >> 4, tj = tid & 15;        // 2x2 Gram blocking
+  const int npairs = a.nb / 2;
+  unsigned* pair_bar = a.bar + kBarrierWords;
+  unsigned* pair_done = pair_bar + npairs;
+  unsigned* blockver = pair_done + npairs;
+  unsigned gen = 0, pair_epoch = 0;
 
   int sweep = 0;
   bool converged = false;
@@ -402,6 +438,11 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         const int c = (tid < JB) ? bi * JB + tid : bj * JB + (tid - JB);
         cols[tid] = (c < a.n) ? c : -1;
       }
+      // both blocks released by the pairs that held them in the previous round
+      if (tid == 0) {
+        while (ld_agent(blockver + bi) < gen || ld_agent(blockver + bj) < gen)
+          __builtin_amdgcn_s_sleep(1);
+      }
       __syncthreads();
       // ---- partial Gram of the pair's columns over this slice's rows ----
       double2 g[2][2];
@@ -409,26 +450,26 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int v = 0; v < 2; ++v) g[u][v] = make_double2(0.0, 0.0);
-      for (int r0 = x_lo; r0 < x_hi; r0 += JR) {
+      for (int gbase = x_lo; gbase < x_hi; gbase += PF * JR) {
+        double2 pre[PF][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = tid + JT * q;
-          const int rr = e % JR, cc = e / JR;
-          const int gr = r0 + rr, gc = cols[cc];
-          Xs[rr][cc] = (gr < x_hi && gc >= 0) ? ld_agent(a.X + (gr + (long)gc * a.ldx))
-                                              : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
+        for (int c = 0; c < PF; ++c) load_chunk(pre[c], a.X, a.ldx, gbase + c * JR, x_hi, cols);
+#pragma unroll
+        for (int c = 0; c < PF; ++c) {
+          if (gbase + c * JR >= x_hi) break;
+          put_chunk(Xs, pre[c]);
+          __syncthreads();
 #pragma unroll 4
-        for (int rr = 0; rr < JR; ++rr) {
-          const double2 a0 = Xs[rr][ti], a1 = Xs[rr][ti + 16];
-          const double2 b0 = Xs[rr][tj], b1 = Xs[rr][tj + 16];
-          cfmac(g[0][0], a0, b0);
-          cfmac(g[0][1], a0, b1);
-          cfmac(g[1][0], a1, b0);
-          cfmac(g[1][1], a1, b1);
+          for (int rr = 0; rr < JR; ++rr) {
+            const double2 a0 = Xs[rr][ti], a1 = Xs[rr][ti + 16];
+            const double2 b0 = Xs[rr][tj], b1 = Xs[rr][tj + 16];
+            cfmac(g[0][0], a0, b0);
+            cfmac(g[0][1], a0, b1);
+            cfmac(g[1][0], a1, b0);
+            cfmac(g[1][1], a1, b1);
+          }
+          __syncthreads();
         }
-        __syncthreads();
       }
       if (R == 1) {
 #pragma unroll
@@ -436,20 +477,39 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
 #pragma unroll
           for (int v = 0; v < 2; ++v) Gs[0][ti + 16 * u][tj + 16 * v] = g[u][v];
       } else {
-        double2* mine = a.gpart + ((long)pair * R + slice) * (JP * JP);
+        double2* gbuf = a.gpart + (long)(gen & 1) * npairs * R * (JP * JP);
+        double2* mine = gbuf + ((long)pair * R + slice) * (JP * JP);
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
           for (int v = 0; v < 2; ++v) st_agent(mine + ((ti + 16 * u) * JP + tj + 16 * v), g[u][v]);
         PHASE(0);
-        grid_barrier(a.bar, nblk, epoch);
+        grid_barrier(pair_bar + pair, R, pair_epoch);
         PHASE(1);
-        // identical summation order in every workgroup of the pair
-        for (int e = tid; e < JP * JP; e += JT) {
-          double2 acc = make_double2(0.0, 0.0);
-          const double2* base = a.gpart + (long)pair * R * (JP * JP) + e;
-          for (int sl = 0; sl < R; ++sl) acc = cadd(acc, ld_agent(base + (long)sl * (JP * JP)));
-          Gs[0][e / JP][e % JP] = acc;
+        // identical summation order (slice 0, 1, ...) in every workgroup of
+        // the pair; loads of four slices are in flight together
+        double2 acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
+        const double2* gp = gbuf + (long)pair * R * (JP * JP);
+        for (int sl0 = 0; sl0 < R; sl0 += 4) {
+          double2 v[4][4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              v[t][q] = (sl0 + t < R) ? ld_agent(gp + (long)(sl0 + t) * (JP * JP) + tid + JT * q)
+                                      : make_double2(0.0, 0.0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (sl0 + t < R) acc[q] = cadd(acc[q], v[t][q]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int e = tid + JT * q;
+          Gs[0][e / JP][e % JP] = acc[q];
         }
       }
       for (int e = tid; e < JP * JP; e += JT)
@@ -547,36 +607,42 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         PHASE(3);
         // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws over this slice ----
         const int rr = tid >> 3, c0 = (tid & 7) * 4;
-        for (int mat = 0; mat < 2; ++mat) {
-          double2* base = mat ? a.V : a.X;
-          const long ld = mat ? a.ldv : a.ldx;
-          const int lo = mat ? v_lo : x_lo, hi = mat ? v_hi : x_hi;
-          for (int r0 = lo; r0 < hi; r0 += JR) {
+        const int nx = (x_hi - x_lo + JR - 1) / JR, nv = (v_hi - v_lo + JR - 1) / JR;
+        for (int t0 = 0; t0 < nx + nv; t0 += PF) {
+          double2 pre[PF][4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int e = tid + JT * q;
-              const int r_ = e % JR, cc = e / JR;
-              const int gr = r0 + r_, gc = cols[cc];
-              Xs[r_][cc] = (gr < hi && gc >= 0) ? ld_agent(base + (gr + (long)gc * ld))
-                                                : make_double2(0.0, 0.0);
-            }
+          for (int c = 0; c < PF; ++c) {
+            const int t = t0 + c;
+            if (t < nx) load_chunk(pre[c], a.X, a.ldx, x_lo + t * JR, x_hi, cols);
+            else if (t < nx + nv) load_chunk(pre[c], a.V, a.ldv, v_lo + (t - nx) * JR, v_hi, cols);
+          }
+#pragma unroll
+          for (int c = 0; c < PF; ++c) {
+            const int t = t0 + c;
+            if (t >= nx + nv) break;
+            const bool isx = t < nx;
+            double2* base = isx ? a.X : a.V;
+            const long ld = isx ? a.ldx : a.ldv;
+            const int r0 = isx ? x_lo + t * JR : v_lo + (t - nx) * JR;
+            const int hi = isx ? x_hi : v_hi;
+            put_chunk(Xs, pre[c]);
             __syncthreads();
             double2 o[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) o[c] = make_double2(0.0, 0.0);
+            for (int cc = 0; cc < 4; ++cc) o[cc] = make_double2(0.0, 0.0);
 #pragma unroll 8
             for (int i = 0; i < JP; ++i) {
               const double2 xv = Xs[rr][i];
 #pragma unroll
-              for (int c = 0; c < 4; ++c) cfma(o[c], xv, Ws[i][c0 + c]);
+              for (int cc = 0; cc < 4; ++cc) cfma(o[cc], xv, Ws[i][c0 + cc]);
             }
             __syncthreads();
             const int gr = r0 + rr;
             if (gr < hi) {
 #pragma unroll
-              for (int c = 0; c < 4; ++c) {
-                const int gc = cols[c0 + c];
-                if (gc >= 0) st_agent(base + (gr + (long)gc * ld), o[c]);
+              for (int cc = 0; cc < 4; ++cc) {
+                const int gc = cols[c0 + cc];
+                if (gc >= 0) st_agent(base + (gr + (long)gc * ld), o[cc]);
               }
             }
           }
@@ -584,9 +650,21 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         if (tid == 0) st_agent(a.flags + sweep, 1);
         PHASE(4);
       }
-      grid_barrier(a.bar, nblk, epoch);
+      // the last slice of the pair to finish releases both blocks
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) {
+        const unsigned old = __hip_atomic_fetch_add(pair_done + pair, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old == (gen + 1) * (unsigned)R - 1) {
+          st_agent(blockver + bi, gen + 1);
+          st_agent(blockver + bj, gen + 1);
+        }
+      }
+      ++gen;
       PHASE(5);
     }
+    grid_barrier(a.bar, nblk, epoch);
     if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT) == 0) {
       converged = true;
