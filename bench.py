@@ -189,6 +189,10 @@ def batch_main(args):
     eng.check(lib.pte_pt_batch_run(batch, args.steps, eng.iptr(more)))
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    counts = [ctypes.c_long() for _ in range(3)]
+    times = [ctypes.c_double() for _ in range(3)]
+    eng.check(lib.pte_pt_batch_stats(batch, *[ctypes.byref(c) for c in counts],
+                                     *[ctypes.byref(t) for t in times]))
     lib.pte_pt_batch_destroy(batch)
     for b in backends:
         b.sync_step()
@@ -220,8 +224,80 @@ def batch_main(args):
                    "one_by_one_instance_steps_per_s": round(
                        count * args.steps / t_single, 4),
                    "batch_speedup": round(t_single / t_batch, 3),
+                   "batches": counts[0].value,
+                   "svds_per_batch": round(counts[1].value
+                                           / max(counts[0].value, 1), 2),
+                   "cooperative_launches": counts[2].value,
+                   "serve_s": round(times[0].value / 1e3, 3),
+                   "batched_qr_s": round(times[1].value / 1e3, 3),
+                   "batched_jacobi_s": round(times[2].value / 1e3, 3),
                    "influence_precompute_s": round(precompute_s, 2)},
         "roofline": None, "cpu_baseline": None}))
+
+
+def tempo_main(args):
+    """TEMPO on the spin-boson configuration with a short memory: H_S =
+    sigma_x / 2, coupling sigma_z / 2, Ohmic alpha 0.3, w_c 3, T 0,
+    rho_0 = |up><up|, dt 0.05, dkmax 40, epsrel 1e-6, 200 steps; the whole
+    step loop is timed (influences precomputed), and the FP64 oracle runs the
+    same steps on the host cores for the baseline and the accuracy."""
+    import numpy as np
+    import torch
+    import oqupy_amd as oqupy
+    from oqupy_amd import operators as op
+    from oqupy_amd.tempo import influence_matrix
+    from oracle.tempo import TempoOracle
+    n, dkmax = 200, 40
+    corr = oqupy.PowerLawSD(alpha=0.3, zeta=1.0, cutoff=3.0,
+                            cutoff_type="exponential", temperature=0.0)
+    bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
+    system = oqupy.System(0.5 * op.sigma("x"))
+    params = oqupy.TempoParameters(dt=0.05, dkmax=dkmax, epsrel=1e-6)
+    rho0 = op.spin_dm("up")
+    table = {dk: influence_matrix(dk, params, bath.correlations,
+                                  bath.coupling_acomm, bath.coupling_comm)
+             for dk in range(dkmax + 1)}
+    tempo = oqupy.Tempo(system, bath, params, rho0, 0.0,
+                        backend_config={"precision": args.precision})
+    tempo._backend_instance._influence = lambda dk: table.get(dk)
+    tempo.compute(end_time=0.05 * 1 + 1e-9)          # initialise + step 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dyn = tempo.compute(end_time=0.05 * n + 1e-9)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    steps = n - 1
+    props = system.get_propagators(0.05, 0.0, params.subdiv_limit,
+                                   params.liouvillian_epsrel)
+    ref = TempoOracle(rho0, lambda dk: table.get(dk), bath.unitary_transform,
+                      props, np.ones(4), np.ones(4), dkmax, 1e-6)
+    _, st = ref.initialize()
+    states = [st.reshape(2, 2)]
+    _, st = ref.compute_step()
+    states.append(st.reshape(2, 2))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ref.compute_step()
+        states.append(st.reshape(2, 2))
+    cpu_elapsed = time.perf_counter() - t0
+    sz = op.sigma("z")
+    dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
+                 - np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
+    print(json.dumps({
+        "metric": "TEMPO time-steps/s (dkmax=40); max |d<sigma_z>| vs FP64 oracle",
+        "value": round(steps / elapsed, 4), "unit": "time-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "spin-boson TEMPO alpha=0.3 wc=3.0 T=0 dt=0.05 "
+                               "dkmax=40 epsrel=1e-6 N=200, steps 2..200",
+                   "max_abs_dsigma_z": float(dsz.max())},
+        "roofline": None,
+        "cpu_baseline": {"value": round(steps / cpu_elapsed, 4),
+                         "unit": "time-steps/s", "cores": blas_threads(),
+                         "kind": "port",
+                         "sample": "FP64 oracle on steps 2..200 of the same run"}}))
 
 
 def main():
@@ -232,6 +308,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of timed oracle work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=("c2", "c1"), default="c2",
+                    help="c1: the TEMPO configuration instead of the headline "
+                         "PT-TEMPO run")
     ap.add_argument("--batch", type=int, default=0,
                     help="time the batched form on this many instances of the "
                          "seeded sweep of baths instead of the headline run")
@@ -244,6 +323,9 @@ def main():
                           "single PT-TEMPO instance is a sequential chain of "
                           "SVDs and is not sharded across GPUs"}))
         sys.exit(2)
+    if args.config == "c1":
+        tempo_main(args)
+        return
     if args.batch > 0:
         if args.batch > 256:
             raise SystemExit("--batch takes at most 256 instances")

@@ -128,6 +128,12 @@ int pte_pt_batch_create(pte_pt_engine* const* engines, int count, pte_pt_batch**
  * receives 1 while engine i has steps left. */
 int pte_pt_batch_run(pte_pt_batch* b, int max_steps, int* more);
 
+/* Served batches and requests (truncated SVDs), cooperative launches, host
+ * milliseconds spent serving, and device milliseconds of the batched QR and
+ * Jacobi launches since the batch was created. */
+int pte_pt_batch_stats(pte_pt_batch* b, long* batches, long* requests, long* launches,
+                       double* serve_ms, double* qr_ms, double* jacobi_ms);
+
 void pte_pt_batch_destroy(pte_pt_batch* b);
 
 /* ---- TEMPO (oqupy/backends/tempo_backend.py:314-626) ---------------------- */
@@ -200,6 +206,18 @@ int pte_dynamics_run_engine(pte_pt_engine* e, const double* props,
                             const double* transform_in,
                             const double* transform_out, const double* rho0,
                             double* states_out);
+
+/* compute_dynamics of nsys systems through the same device-resident process
+ * tensor (the same state vector sizes and transforms as
+ * pte_dynamics_run_engine): props holds nsys x num_steps x 2 x d^2 x d^2,
+ * rho0 nsys x d^2, states_out receives nsys x (num_steps + 1) x d^2. The
+ * process tensor is read by all systems in the same kernels. No counterpart
+ * in the reference; each system's result equals compute_dynamics
+ * (oqupy/system_dynamics.py:41-182) on its own. */
+int pte_dynamics_run_engine_batch(pte_pt_engine* e, int nsys, const double* props,
+                                  const double* transform_in,
+                                  const double* transform_out, const double* rho0,
+                                  double* states_out);
 
 /* Truncated SVD of one matrix by the engine's kernels: M (m x n, row-major)
  * = U diag(S) Vh with the kept rank k = #{j : sqrt(sum_{i>=j} s_i^2) >

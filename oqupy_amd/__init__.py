@@ -15,7 +15,7 @@ from oqupy_amd.process_tensor import (SimpleProcessTensor,
                                       TrivialProcessTensor)
 from oqupy_amd.pt_tempo import PtTempo, pt_tempo_compute, pt_tempo_compute_batch
 from oqupy_amd.system import System, TimeDependentSystem
-from oqupy_amd.system_dynamics import compute_dynamics
+from oqupy_amd.system_dynamics import compute_dynamics, compute_dynamics_batch
 from oqupy_amd.tempo import (Tempo, TempoParameters, influence_matrix,
                              tempo_compute)
 from oqupy_amd.version import __version__
@@ -24,6 +24,7 @@ __all__ = [
     'Bath', 'CustomCorrelations', 'CustomSD', 'Dynamics', 'PowerLawSD',
     'PtTempo', 'SimpleProcessTensor', 'System', 'Tempo', 'TempoParameters',
     'TimeDependentSystem', 'TrivialProcessTensor', 'compute_dynamics',
+    'compute_dynamics_batch',
     'influence_matrix', 'operators', 'pt_tempo_compute',
     'pt_tempo_compute_batch', 'tempo_compute',
     '__version__',

@@ -103,7 +103,11 @@ _SIGNATURES = {
     "pte_pt_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int,
                                            ctypes.POINTER(_P)]),
     "pte_pt_batch_run": (ctypes.c_int, [_P, ctypes.c_int, _IP]),
+    "pte_pt_batch_stats": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_long)] * 3
+                           + [_DP] * 3),
     "pte_pt_batch_destroy": (None, [_P]),
+    "pte_dynamics_run_engine_batch": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP,
+                                                     _DP, _DP, _DP]),
 }
 
 PRECISIONS = {"c128": 0, "c64": 1}

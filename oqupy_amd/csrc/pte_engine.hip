@@ -210,6 +210,7 @@ struct Workspace {
   int jacobi_sync_words = kBarrierWords;   // words of `bar` the last prepared Jacobi uses
   int qr_blocks = 256;       // tuning: PTE_QR_BLOCKS (workgroups of the pivoted QR)
   int slice_rows = 64;       // tuning: PTE_JACOBI_SLICE_ROWS (min rows per slice)
+  bool small_enabled = true; // tuning: PTE_SMALL_SVD=0 sends every SVD to the grid path
   int max_coop_blocks = 0;
   double epsrel = 1e-7;
   int chi_cap = 512;
@@ -266,6 +267,13 @@ struct Workspace {
     HIP_CHECK(hipEventCreate(&evq1));
     if (const char* v = std::getenv("PTE_JACOBI_INNER_SWEEPS"))
       inner_sweeps = std::max(1, std::min(8, std::atoi(v)));
+    if (const char* v = std::getenv("PTE_SMALL_SVD")) small_enabled = std::atoi(v) != 0;
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kSmallLdsMax));
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_batch_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kSmallLdsMax));
     if (const char* v = std::getenv("PTE_JACOBI_SLICE_ROWS"))
       slice_rows = std::max(32, std::min(4096, std::atoi(v)));
     if (const char* v = std::getenv("PTE_QR_BLOCKS"))
@@ -450,6 +458,7 @@ struct Workspace {
   // Returns the kept rank k.
   int svd_qrcp(int m, int n) {
     if (batcher) return batched_svd(true, m, n);
+    if (small_ok(m, n)) return svd_small(m, n);
     const int nblk = qr_natural_grid(n);
     QrcpArgs qa = qr_prepare(m, n, nblk);
     HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
@@ -466,6 +475,60 @@ struct Workspace {
     HIP_CHECK(hipEventElapsedTime(&ms, evq0, evq1));
     const int r = qr_finish(m, n, ms);
     return svd(n, r);
+  }
+
+  // ---- small SVDs: one workgroup, LDS resident (small_svd_kernel) ----
+  static constexpr size_t kSmallLdsMax = 150 * 1024;
+  static size_t small_lds(int m, int n) {
+    const size_t r = (size_t)std::min(m, n);
+    return std::max((size_t)m * n * 16 + (size_t)n * 20, ((size_t)n * r + r * r) * 16);
+  }
+  bool small_ok(int m, int n) const {
+    return small_enabled && n <= kSmallMaxCols && small_lds(m, n) <= kSmallLdsMax;
+  }
+
+  // buffers and arguments of small_svd_kernel on Q (m x n)
+  SmallSvdArgs small_prepare(int m, int n, SvdInfo* copy) {
+    require(m >= 1 && n >= 1, PTE_ERR_ARG, "internal: empty SVD");
+    const int rmax = std::min(m, n);
+    ensure_qr((size_t)n);
+    ensure_sv((size_t)rmax);
+    X.ensure((size_t)n * rmax, svd_stream);
+    V.ensure((size_t)rmax * rmax, svd_stream);
+    SmallSvdArgs sa;
+    sa.A = Q.p; sa.m = m; sa.n = n;
+    sa.tau = qr_tau; sa.vscale = qr_vscale; sa.beta = qr_beta; sa.piv = qr_piv;
+    sa.qrank = qr_rank_d;
+    sa.Y = X.p; sa.W = V.p;
+    sa.sigma = sigma; sa.perm = perm; sa.sorted = sorted; sa.info = info_d;
+    sa.epsrel = epsrel; sa.max_rank = chi_cap; sa.max_sweeps = max_sweeps;
+    sa.copy = copy;
+    return sa;
+  }
+
+  // after synchronisation (summary in info_h): the same checks, statistics
+  // and state as the pivoted-QR + Jacobi path
+  int small_finish(int m, int n, float ms) {
+    const int r = info_h->pad;
+    require(r >= 1, PTE_ERR_NUMERIC, "SVD of a zero matrix");
+    qr_count += 1;
+    tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = 0.f;
+    qr_m = m; qr_n = n; qr_r = r;
+    return jacobi_check(n, r, ms);
+  }
+
+  int svd_small(int m, int n) {
+    SmallSvdArgs sa = small_prepare(m, n, nullptr);
+    HIP_CHECK(hipEventRecord(evj0, svd_stream));
+    small_svd_kernel<<<1, 256, small_lds(m, n), svd_stream>>>(sa);
+    HIP_CHECK(hipEventRecord(evj1, svd_stream));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
+                             svd_stream));
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
+    return small_finish(m, n, ms);
   }
 
   int qr_natural_grid(int n) const { return std::max(1, std::min({n, qr_blocks, num_cus})); }
@@ -616,6 +679,9 @@ struct Batcher {
   std::vector<SvdRequest*> pending;
   int max_jacobi_grid = 0;
   int num_cus = 0;
+  // statistics: served batches, requests, launches, host wall time serving
+  long n_batches = 0, n_requests = 0, n_launches = 0;
+  double serve_ms = 0.0, qr_ms_total = 0.0, jac_ms_total = 0.0;
   // argument arrays: pinned staging and device copies
   char* stage_h = nullptr;
   char* stage_d = nullptr;
@@ -633,6 +699,9 @@ struct Batcher {
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, reinterpret_cast<const void*>(jacobi_batch_kernel), 256, 0));
     max_jacobi_grid = std::min(per_cu * num_cus, 1024);
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_batch_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)Workspace::kSmallLdsMax));
     require(max_jacobi_grid > 0, PTE_ERR_DEVICE, "batched Jacobi kernel cannot be resident");
   }
 
@@ -690,6 +759,9 @@ struct Batcher {
     std::vector<SvdRequest*> batch;
     batch.swap(pending);
     lk.unlock();
+    const auto t0 = std::chrono::steady_clock::now();
+    n_batches += 1;
+    n_requests += (long)batch.size();
     try {
       execute(batch);
     } catch (...) {
@@ -698,6 +770,8 @@ struct Batcher {
         if (!r->failed) { r->failed = true; r->err = std::current_exception(); }
       }
     }
+    serve_ms += std::chrono::duration<double, std::milli>(
+                    std::chrono::steady_clock::now() - t0).count();
     lk.lock();
     for (auto* r : batch) r->done = true;
     cv.notify_all();
@@ -798,6 +872,48 @@ struct Batcher {
     gather_cap = c;
   }
 
+  // small SVDs: one workgroup each, all in one launch
+  void run_small(std::vector<SvdRequest*>& rs) {
+    ensure_gather(rs.size());
+    std::vector<SmallSvdArgs> args;
+    std::vector<SvdRequest*> in;
+    size_t lds = 0;
+    for (auto* r : rs) {
+      try {
+        args.push_back(r->ws->small_prepare(r->m, r->n, infos_d + in.size()));
+        lds = std::max(lds, Workspace::small_lds(r->m, r->n));
+        in.push_back(r);
+      } catch (...) {
+        fail(r);
+      }
+    }
+    if (in.empty()) return;
+    Pack pk;
+    const size_t oa = pk.add(args.data(), args.size() * sizeof(SmallSvdArgs));
+    char* base = upload(pk);
+    HIP_CHECK(hipEventRecord(ev0, stream));
+    small_svd_batch_kernel<<<(unsigned)in.size(), 256, lds, stream>>>(
+        reinterpret_cast<const SmallSvdArgs*>(base + oa));
+    HIP_CHECK(hipEventRecord(ev1, stream));
+    HIP_CHECK(hipGetLastError());
+    n_launches += 1;
+    HIP_CHECK(hipMemcpyAsync(infos_h, infos_d, in.size() * sizeof(SvdInfo),
+                             hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    jac_ms_total += ms;
+    for (size_t i = 0; i < in.size(); ++i) {
+      SvdRequest* r = in[i];
+      try {
+        *r->ws->info_h = infos_h[i];
+        r->k = r->ws->small_finish(r->m, r->n, ms);
+      } catch (...) {
+        fail(r);
+      }
+    }
+  }
+
   void run_qr(std::vector<SvdRequest*>& qrs) {
     int max_m = 0;
     for (auto* r : qrs) max_m = std::max(max_m, r->m);
@@ -853,6 +969,7 @@ struct Batcher {
       HIP_CHECK(hipLaunchCooperativeKernel(
           reinterpret_cast<const void*>(qrcp_batch_kernel), dim3(total), dim3(256), kargs,
           lds, stream));
+      n_launches += 1;
       HIP_CHECK(hipEventRecord(ev1, stream));
       gather_qr_rank_kernel<<<1, 256, 0, stream>>>(da, (int)in.size(), ranks_d);
       HIP_CHECK(hipMemcpyAsync(ranks_h, ranks_d, in.size() * sizeof(int),
@@ -860,6 +977,7 @@ struct Batcher {
       HIP_CHECK(hipStreamSynchronize(stream));
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      qr_ms_total += ms;
       std::vector<RhArgs> rh;
       long max_elems = 0;
       for (size_t i = 0; i < in.size(); ++i) {
@@ -942,6 +1060,7 @@ struct Batcher {
       HIP_CHECK(hipLaunchCooperativeKernel(
           reinterpret_cast<const void*>(jacobi_batch_kernel), dim3(total), dim3(256), kargs,
           0, stream));
+      n_launches += 1;
       HIP_CHECK(hipEventRecord(ev1, stream));
       const unsigned np = (unsigned)in.size();
       column_norm_batch_kernel<<<dim3(max_n, np), 256, 0, stream>>>(dp);
@@ -953,6 +1072,7 @@ struct Batcher {
       HIP_CHECK(hipStreamSynchronize(stream));
       float ms = 0.f;
       HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      jac_ms_total += ms;
       for (size_t i = 0; i < in.size(); ++i) {
         SvdRequest* r = in[i];
         try {
@@ -970,15 +1090,17 @@ struct Batcher {
       HIP_CHECK(hipStreamWaitEvent(stream, r->ws->ev_ready, 0));
       r->ws->svd_stream = stream;
     }
-    std::vector<SvdRequest*> qrs;
+    std::vector<SvdRequest*> qrs, smalls;
     for (auto* r : batch) {
-      if (r->qr) qrs.push_back(r);
+      if (r->qr && r->ws->small_ok(r->m, r->n)) smalls.push_back(r);
+      else if (r->qr) qrs.push_back(r);
       else { r->jm = r->m; r->jn = r->n; }
     }
+    if (!smalls.empty()) run_small(smalls);
     if (!qrs.empty()) run_qr(qrs);
     std::vector<SvdRequest*> jr;
     for (auto* r : batch)
-      if (!r->failed) jr.push_back(r);
+      if (!r->failed && !(r->qr && r->ws->small_ok(r->m, r->n))) jr.push_back(r);
     if (!jr.empty()) run_jacobi(jr);
     for (auto* r : batch) r->ws->svd_stream = r->ws->stream;
   }
@@ -1732,12 +1854,19 @@ struct TempoEngine {
 };
 
 // ---------------------------------------------------------------------------
-// compute_dynamics kernels (system_dynamics.py:631-700)
+// compute_dynamics kernels (system_dynamics.py:631-700), for S systems
+// (grid y) through one process tensor; v, w are [S][bond][d2], propagators
+// [S][N][2][d2][d2]
 // ---------------------------------------------------------------------------
-// w[a][i] = sum_j (sum_q v[a][q] P1[j][q]) Tin[j][i]  (Tin = identity if null)
+
+// w[a][i] = sum_j (P1 v[a])[j] Tin[j][i]   (Tin = identity when null)
 __global__ void __launch_bounds__(256)
-dyn_pre_kernel(const double2* v, int A, int d2, const double2* P1, const double2* tin,
-               double2* w) {
+dyn_pre_kernel(const double2* v, int A, int d2, long vstride, const double2* P1,
+               long pstride, const double2* tin, double2* w) {
+  const int sys = blockIdx.y;
+  v += sys * vstride;
+  w += sys * vstride;
+  P1 += sys * pstride;
   const long total = (long)A * d2;
   for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
     const int i = e % d2;
@@ -1757,10 +1886,15 @@ dyn_pre_kernel(const double2* v, int A, int d2, const double2* P1, const double2
 // then v''[b][o2] = sum_o P2[o2][o] v'[b][o].  One thread per b; the site is
 // read in its [a][i][b] layout (coalesced over b).
 __global__ void __launch_bounds__(256)
-dyn_mpo_kernel(const double2* w, const double2* T, int A, int d2, int B, double scale,
-               const double2* tout, const double2* P2, double2* vout) {
+dyn_mpo_kernel(const double2* w, const double2* T, int A, int d2, int B, long vstride,
+               double scale, const double2* tout, const double2* P2, long pstride,
+               double2* vout) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= B) return;
+  const int sys = blockIdx.y;
+  w += sys * vstride;
+  vout += sys * vstride;
+  P2 += sys * pstride;
   double2 u[16];
   for (int i = 0; i < d2; ++i) u[i] = make_double2(0.0, 0.0);
   for (int a = 0; a < A; ++a)
@@ -1782,10 +1916,14 @@ dyn_mpo_kernel(const double2* w, const double2* T, int A, int d2, int B, double 
   }
 }
 
-// rho[o] = sum_a cap[a] v[a][o]   (single workgroup)
+// rho[o] = sum_a cap[a] v[a][o]   (one workgroup per system)
 __global__ void __launch_bounds__(256)
-dyn_readout_kernel(const double2* v, const double2* cap, int A, int d2, double2* rho) {
+dyn_readout_kernel(const double2* v, const double2* cap, int A, int d2, long vstride,
+                   double2* rho, long rstride) {
   __shared__ double2 red[256];
+  const int sys = blockIdx.y;
+  v += sys * vstride;
+  rho += sys * rstride;
   for (int o = 0; o < d2; ++o) {
     double2 acc = make_double2(0.0, 0.0);
     for (int a = threadIdx.x; a < A; a += 256) cfma(acc, cap[a], v[(long)a * d2 + o]);
@@ -1800,47 +1938,53 @@ dyn_readout_kernel(const double2* v, const double2* cap, int A, int d2, double2*
   }
 }
 
-// Runs the dynamics over device-resident sites [a][i][b] (scaled by `scale`)
-// and caps.
+// Runs the dynamics of S systems over device-resident sites [a][i][b]
+// (scaled by `scale`) and caps; props: S x N x 2 x d2 x d2, rho0: S x d2,
+// states_out: S x (N+1) x d2 (interleaved complex).
 static void run_dynamics(hipStream_t st, int d2, int N, const std::vector<const double2*>& sites,
                          const std::vector<int>& bonds, const std::vector<const double2*>& caps,
                          const double* props, const double* tin_h, const double* tout_h,
-                         const double* rho0, double scale, double* states_out) {
+                         const double* rho0, double scale, double* states_out, int S = 1) {
   require(d2 >= 1 && d2 <= 16, PTE_ERR_ARG, "Liouville dimension must be 1..16");
-  const long pstride = 2L * d2 * d2;
+  require(S >= 1 && S <= 65535, PTE_ERR_ARG, "number of systems must be 1..65535");
+  const long pstride = 2L * N * d2 * d2;      // per system, complex entries
   auto up = [&](const double* h, long count) {
     double2* p = nullptr;
     HIP_CHECK(hipMallocAsync((void**)&p, std::max<long>(count, 1) * sizeof(double2), st));
     HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(double2), hipMemcpyHostToDevice, st));
     return p;
   };
-  for (long i = 0; i < 2L * N * d2 * d2 * 2; ++i)
+  for (long i = 0; i < (long)S * pstride * 2; ++i)
     require(std::isfinite(props[i]), PTE_ERR_NUMERIC, "non-finite propagator");
-  double2* dprops = up(props, 2L * N * d2 * d2);
+  double2* dprops = up(props, (long)S * pstride);
   double2* tin = tin_h ? up(tin_h, (long)d2 * d2) : nullptr;
   double2* tout = tout_h ? up(tout_h, (long)d2 * d2) : nullptr;
   int maxb = 1;
   for (int b : bonds) maxb = std::max(maxb, b);
+  const long vstride = (long)maxb * d2, rstride = (long)(N + 1) * d2;
   double2 *v = nullptr, *v2 = nullptr, *w = nullptr, *rho = nullptr;
-  HIP_CHECK(hipMallocAsync((void**)&v, (size_t)maxb * d2 * sizeof(double2), st));
-  HIP_CHECK(hipMallocAsync((void**)&v2, (size_t)maxb * d2 * sizeof(double2), st));
-  HIP_CHECK(hipMallocAsync((void**)&w, (size_t)maxb * d2 * sizeof(double2), st));
-  HIP_CHECK(hipMallocAsync((void**)&rho, (size_t)(N + 1) * d2 * sizeof(double2), st));
-  HIP_CHECK(hipMemcpyAsync(v, rho0, d2 * sizeof(double2), hipMemcpyHostToDevice, st));
-  (void)pstride;
+  HIP_CHECK(hipMallocAsync((void**)&v, (size_t)S * vstride * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&v2, (size_t)S * vstride * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&w, (size_t)S * vstride * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&rho, (size_t)S * rstride * sizeof(double2), st));
+  HIP_CHECK(hipMemcpy2DAsync(v, vstride * sizeof(double2), rho0, d2 * sizeof(double2),
+                             d2 * sizeof(double2), S, hipMemcpyHostToDevice, st));
   for (int k = 0; k < N; ++k) {
     const int A = bonds[k], B = bonds[k + 1];
-    dyn_readout_kernel<<<1, 256, 0, st>>>(v, caps[k], A, d2, rho + (long)k * d2);
+    dyn_readout_kernel<<<dim3(1, S), 256, 0, st>>>(v, caps[k], A, d2, vstride,
+                                                   rho + (long)k * d2, rstride);
     const double2* P1 = dprops + (long)k * 2 * d2 * d2;
     const double2* P2 = P1 + d2 * d2;
-    dyn_pre_kernel<<<grid_for((long)A * d2), 256, 0, st>>>(v, A, d2, P1, tin, w);
-    dyn_mpo_kernel<<<(B + 255) / 256, 256, 0, st>>>(w, sites[k], A, d2, B, scale, tout, P2,
-                                                     v2);
+    dyn_pre_kernel<<<dim3(grid_for((long)A * d2), S), 256, 0, st>>>(v, A, d2, vstride, P1,
+                                                                    pstride, tin, w);
+    dyn_mpo_kernel<<<dim3((B + 255) / 256, S), 256, 0, st>>>(w, sites[k], A, d2, B, vstride,
+                                                             scale, tout, P2, pstride, v2);
     std::swap(v, v2);
   }
-  dyn_readout_kernel<<<1, 256, 0, st>>>(v, caps[N], bonds[N], d2, rho + (long)N * d2);
+  dyn_readout_kernel<<<dim3(1, S), 256, 0, st>>>(v, caps[N], bonds[N], d2, vstride,
+                                                 rho + (long)N * d2, rstride);
   HIP_CHECK(hipGetLastError());
-  HIP_CHECK(hipMemcpyAsync(states_out, rho, (size_t)(N + 1) * d2 * sizeof(double2),
+  HIP_CHECK(hipMemcpyAsync(states_out, rho, (size_t)S * rstride * sizeof(double2),
                            hipMemcpyDeviceToHost, st));
   HIP_CHECK(hipStreamSynchronize(st));
   (void)hipFreeAsync(v, st); (void)hipFreeAsync(v2, st); (void)hipFreeAsync(w, st); (void)hipFreeAsync(rho, st);
@@ -2092,6 +2236,20 @@ int pte_pt_batch_run(pte_pt_batch* b, int max_steps, int* more) {
   });
 }
 
+int pte_pt_batch_stats(pte_pt_batch* b, long* batches, long* requests, long* launches,
+                       double* serve_ms, double* qr_ms, double* jacobi_ms) {
+  return guarded([&] {
+    require(b && batches && requests && launches && serve_ms && qr_ms && jacobi_ms,
+            PTE_ERR_ARG, "null argument");
+    *batches = b->batcher.n_batches;
+    *requests = b->batcher.n_requests;
+    *launches = b->batcher.n_launches;
+    *serve_ms = b->batcher.serve_ms;
+    *qr_ms = b->batcher.qr_ms_total;
+    *jacobi_ms = b->batcher.jac_ms_total;
+  });
+}
+
 void pte_pt_batch_destroy(pte_pt_batch* b) {
   if (!b) return;
   guarded([&] { b->batcher.destroy(); });
@@ -2331,6 +2489,29 @@ int pte_dynamics_run_engine(pte_pt_engine* e, const double* props, const double*
     for (int k = 0; k <= N; ++k) caps[k] = pe.caps[k].p;
     run_dynamics(pe.st(), d2, N, sites, b, caps, props, tin, tout, rho0,
                  (double)pe.cfg.dimension, states_out);
+  });
+}
+
+int pte_dynamics_run_engine_batch(pte_pt_engine* e, int nsys, const double* props,
+                                  const double* tin, const double* tout, const double* rho0,
+                                  double* states_out) {
+  return guarded([&] {
+    require(e && props && rho0 && states_out && nsys >= 1, PTE_ERR_ARG, "bad arguments");
+    PtEngine& pe = e->impl;
+    require(pe.step >= pe.cfg.num_steps, PTE_ERR_STATE, "process tensor not complete");
+    require(!pe.caps.empty(), PTE_ERR_STATE, "caps not computed");
+    const int N = pe.cfg.num_steps, d2 = pe.cfg.dimension * pe.cfg.dimension;
+    std::vector<int> b(N + 1);
+    std::vector<const double2*> sites(N), caps(N + 1);
+    for (int k = 0; k < N; ++k) {
+      b[k] = pe.mps.sites[k].d[0];
+      sites[k] = pe.mps.sites[k].p;
+      require(pe.mps.sites[k].d[1] == d2, PTE_ERR_ARG, "internal: site leg is not d^2");
+    }
+    b[N] = pe.mps.sites[N - 1].d[2];
+    for (int k = 0; k <= N; ++k) caps[k] = pe.caps[k].p;
+    run_dynamics(pe.st(), d2, N, sites, b, caps, props, tin, tout, rho0,
+                 (double)pe.cfg.dimension, states_out, nsys);
   });
 }
 

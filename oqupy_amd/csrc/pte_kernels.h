@@ -113,6 +113,21 @@ struct RhArgs {
   double2* Y;
 };
 
+// One small truncated SVD done by one workgroup (small_svd_kernel): the same
+// outputs as the pivoted QR + Jacobi path (reflectors in A, tau, vscale,
+// beta, piv, *qrank; Z = Y W in Y (n x r, ld n); W (r x r, ld r); sigma,
+// perm, sorted; info with info->pad = r).
+struct SmallSvdArgs {
+  double2* A; int m, n;
+  double2* tau; double2* vscale; double* beta; int* piv; int* qrank;
+  double2* Y; double2* W;
+  double* sigma; int* perm; double* sorted; SvdInfo* info;
+  double epsrel; int max_rank; int max_sweeps;
+  SvdInfo* copy;    // optional second copy of the summary (batched gather)
+};
+
+constexpr int kSmallMaxCols = 64;           // columns of M in small_svd_kernel
+
 struct ZeroItem {
   unsigned* p; int words; int pad;
 };
@@ -127,6 +142,8 @@ __global__ void column_norm_batch_kernel(const SvdPostArgs* pa);
 __global__ void rank_sort_batch_kernel(const SvdPostArgs* pa);
 __global__ void truncate_batch_kernel(const SvdPostArgs* pa, SvdInfo* out);
 __global__ void zero_items_kernel(const ZeroItem* items);
+__global__ void small_svd_kernel(SmallSvdArgs a);
+__global__ void small_svd_batch_kernel(const SmallSvdArgs* probs);
 __global__ void gather_qr_rank_kernel(const QrcpArgs* probs, int n, int* out);
 __global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);

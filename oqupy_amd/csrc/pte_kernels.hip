@@ -1099,6 +1099,293 @@ build_rh_batch_kernel(const RhArgs* pa) {
   build_rh_body(a.A, a.lda, a.n, a.piv, a.beta, a.rank, a.Y, blockIdx.x, gridDim.x);
 }
 
+// ---------------------------------------------------------------------------
+// Small truncated SVD in one workgroup (M with at most kSmallMaxCols columns,
+// staged in LDS): the pivoted QR of qrcp_kernel (same reflectors, pivoting,
+// norm downdating and eps-rank stop), then the one-sided Jacobi of
+// Y = R^H (n x r) with scalar column pairs in the round-robin order (the
+// same rotation formulas and stopping rule as jacobi_kernel), then singular
+// values, order and cut. Barriers are workgroup barriers only; column pairs
+// (and QR columns) are reduced by lane groups with wavefront shuffles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double group_sum(double v, int G) {
+  for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __shfl(v, (threadIdx.x & 63) & ~(G - 1), 64);
+}
+
+__device__ __forceinline__ int pow2_ceil(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+__device__ void small_svd_body(const SmallSvdArgs& a) {
+  extern __shared__ double2 lds_small[];
+  __shared__ double slot[2][4];
+  __shared__ int s_piv;
+  __shared__ double s_total;
+  int parity = 0;
+  const int m = a.m, n = a.n, tid = threadIdx.x, lane = tid & 63;
+  double2* As = lds_small;                                  // m x n, ld m
+  double* vn1 = reinterpret_cast<double*>(As + (long)m * n);
+  double* vn2 = vn1 + n;
+  int* done = reinterpret_cast<int*>(vn2 + n);
+  const double eps = 2.220446049250313e-16;
+
+  for (long e = tid; e < (long)m * n; e += 256) As[e] = a.A[e];
+  __syncthreads();
+  // ---- pivoted QR in LDS ----
+  const int NP = pow2_ceil(n);
+  const int G = NP >= 4 ? 256 / NP : 64;         // threads per column (<= 64)
+  const int myc = tid / G, g = tid % G;
+  {
+    double part = 0.0;
+    if (myc < n)
+      for (int i = g; i < m; i += G) part += cabs2(As[i + (long)myc * m]);
+    const double nrm = sqrt(group_sum(part, G));
+    if (myc < n && g == 0) { vn1[myc] = nrm; vn2[myc] = nrm; done[myc] = 0; }
+  }
+  __syncthreads();
+  auto choose = [&]() {
+    // wavefront 0: pivot (largest residual norm, ties to the smaller column)
+    // and the residual Frobenius norm squared
+    if (tid < 64) {
+      double bv = -1.0, sv = 0.0;
+      int bc = -1;
+      for (int c = lane; c < n; c += 64)
+        if (!done[c]) { sv += vn1[c] * vn1[c]; better(bv, bc, vn1[c], c); }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oc = __shfl_xor(bc, off, 64);
+        better(bv, bc, ov, oc);
+      }
+      sv = wave_sum(sv);
+      if (lane == 0) { s_piv = bc; s_total = sv; }
+    }
+    __syncthreads();
+  };
+  choose();
+  const double drop2 = eps * eps * s_total;
+  const double tol3z = 1.4901161193847656e-08;
+  const int kmax = m < n ? m : n;
+  int j = 0;
+  while (j < kmax && s_piv >= 0 && s_total > drop2) {
+    const int p = s_piv;
+    const double2* pc = As + (long)p * m;
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += 256) part += cabs2(pc[i]);
+    const double xnorm2 = block_sum256(part, slot, parity);
+    const double2 alpha = pc[j];
+    double2 tau, scale;
+    double beta;
+    if (xnorm2 == 0.0 && alpha.y == 0.0) {
+      tau = make_double2(0.0, 0.0);
+      scale = make_double2(0.0, 0.0);
+      beta = alpha.x;
+    } else {
+      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
+      const double dr = alpha.x - beta, di = alpha.y;
+      const double den = dr * dr + di * di;
+      scale = make_double2(dr / den, -di / den);
+      tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    }
+    if (tid == 0) {
+      a.piv[j] = p;
+      a.tau[j] = tau;
+      a.vscale[j] = scale;
+      a.beta[j] = beta;
+      done[p] = 1;
+    }
+    __syncthreads();
+    const bool reflect = (tau.x != 0.0 || tau.y != 0.0);
+    if (myc < n && !done[myc]) {
+      double2* col = As + (long)myc * m;
+      if (reflect) {
+        double2 w = make_double2(0.0, 0.0);
+        for (int i = j + g; i < m; i += G) {
+          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+          cfmac(w, vi, col[i]);
+        }
+        w.x = group_sum(w.x, G);
+        w.y = group_sum(w.y, G);
+        const double2 tw = cmul(make_double2(tau.x, -tau.y), w);
+        for (int i = j + g; i < m; i += G) {
+          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+          col[i] = csub(col[i], cmul(vi, tw));
+        }
+      }
+    }
+    __syncthreads();
+    if (myc < n && !done[myc]) {
+      const double2* col = As + (long)myc * m;
+      double nv = vn1[myc];
+      if (nv != 0.0) {
+        const double rj = sqrt(cabs2(col[j]));
+        double temp = 1.0 - (rj / nv) * (rj / nv);
+        temp = temp > 0.0 ? temp : 0.0;
+        const double ratio = nv / vn2[myc];
+        if (temp * ratio * ratio <= tol3z) {
+          double pp = 0.0;
+          for (int i = j + 1 + g; i < m; i += G) pp += cabs2(col[i]);
+          nv = (j + 1 < m) ? sqrt(group_sum(pp, G)) : 0.0;
+          if (g == 0) { vn1[myc] = nv; vn2[myc] = nv; }
+        } else if (g == 0) {
+          vn1[myc] = nv * sqrt(temp);
+        }
+      }
+    }
+    __syncthreads();
+    choose();
+    ++j;
+  }
+  const int r = j;
+  if (tid == 0) {
+    int t = r;
+    for (int c = 0; c < n; ++c)
+      if (!done[c]) a.piv[t++] = c;
+    *a.qrank = r;
+  }
+  // reflectors and R back to global memory (apply_q_kernel reads them)
+  for (long e = tid; e < (long)m * n; e += 256) a.A[e] = As[e];
+  __syncthreads();
+  if (r == 0) {
+    if (tid == 0) {
+      SvdInfo inf;
+      inf.rank = 0; inf.overflow = 0; inf.sweeps = 1; inf.pad = 0;
+      inf.s0 = 0.0; inf.discarded = 0.0;
+      *a.info = inf;
+      if (a.copy) *a.copy = inf;
+    }
+    return;
+  }
+  // ---- Y = R^H (n x r, ld n) and W = I (r x r) in LDS ----
+  double2* Ys = lds_small;
+  double2* Wsm = Ys + (long)n * r;
+  for (long e = tid; e < (long)n * r; e += 256) {
+    const int sidx = e % n, t = e / n;
+    const int ps = a.piv[sidx];
+    double2 v = make_double2(0.0, 0.0);
+    if (t < sidx) v = cconj(a.A[t + (long)ps * m]);   // As is being overwritten
+    else if (t == sidx) v = make_double2(a.beta[sidx], 0.0);
+    Ys[e] = v;
+  }
+  __syncthreads();
+  for (long e = tid; e < (long)r * r; e += 256)
+    Wsm[e] = make_double2((e % r) == (e / r) ? 1.0 : 0.0, 0.0);
+  double fpart = 0.0;
+  for (long e = tid; e < (long)n * r; e += 256) fpart += cabs2(Ys[e]);
+  const double fro = sqrt(block_sum256(fpart, slot, parity));
+  const double tol = (n > 16 ? n : 16) * eps, abs_tol = eps * fro;
+  const double tol2 = tol * tol, abs_tol2 = abs_tol * abs_tol;
+  // ---- one-sided Jacobi on the columns of Y ----
+  const int re = r + (r & 1);
+  const int P = re / 2;
+  const int PG = pow2_ceil(P);
+  const int GJ = PG >= 4 ? 256 / PG : 64;         // threads per pair (<= 64)
+  const int q = tid / GJ, gj = tid % GJ;
+  int sweeps = 0;
+  bool converged = false;
+  for (; sweeps < a.max_sweeps; ++sweeps) {
+    int any = 0;
+    for (int round = 0; round < (re > 1 ? re - 1 : 1); ++round) {
+      int ci = -1, cj = -1;
+      if (q < P && re > 1) {
+        tournament_pair(re, round, q, ci, cj);
+        if (ci > cj) { const int tt = ci; ci = cj; cj = tt; }
+        if (cj >= r) { ci = -1; cj = -1; }
+      }
+      if (ci >= 0) {
+        double2* yi = Ys + (long)ci * n;
+        double2* yj = Ys + (long)cj * n;
+        double al = 0.0, be = 0.0;
+        double2 ga = make_double2(0.0, 0.0);
+        for (int i = gj; i < n; i += GJ) {
+          al += cabs2(yi[i]);
+          be += cabs2(yj[i]);
+          cfmac(ga, yi[i], yj[i]);
+        }
+        al = group_sum(al, GJ);
+        be = group_sum(be, GJ);
+        ga.x = group_sum(ga.x, GJ);
+        ga.y = group_sum(ga.y, GJ);
+        double rot[4];
+        if (jacobi_rotation(al, be, ga, tol2, abs_tol2, rot)) {
+          any = 1;
+          const double c = rot[0], sn = rot[1];
+          const double2 cph = make_double2(rot[2], -rot[3]);
+          for (int i = gj; i < n; i += GJ) {
+            const double2 xp = yi[i];
+            const double2 xq = cmul(yj[i], cph);
+            yi[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+            yj[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+          }
+          double2* wi = Wsm + (long)ci * r;
+          double2* wj = Wsm + (long)cj * r;
+          for (int i = gj; i < r; i += GJ) {
+            const double2 xp = wi[i];
+            const double2 xq = cmul(wj[i], cph);
+            wi[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+            wj[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (!__syncthreads_or(any)) {
+      converged = true;
+      break;
+    }
+  }
+  // ---- singular values, order, cut; Z and W to global memory ----
+  for (long e = tid; e < (long)n * r; e += 256) a.Y[e] = Ys[e];
+  for (long e = tid; e < (long)r * r; e += 256) a.W[e] = Wsm[e];
+  for (int c = tid / 64; c < r; c += 4) {
+    double part2 = 0.0;
+    for (int i = lane; i < n; i += 64) part2 += cabs2(Ys[i + (long)c * n]);
+    const double sg = sqrt(wave_sum(part2));
+    if (lane == 0) a.sigma[c] = sg;
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (tid < r) rank_sort_body(a.sigma, r, a.perm, a.sorted, tid);
+  __threadfence_block();
+  __syncthreads();
+  if (tid == 0) {
+    // smallest k with sqrt(sum_{i>=k} s_i^2) <= epsrel s_0 (suffix sums)
+    const double s0 = a.sorted[0];
+    const double thr = a.epsrel * s0;
+    double tail = 0.0;
+    int k = 0;
+    for (int i = r - 1; i >= 0; --i) {
+      tail += a.sorted[i] * a.sorted[i];
+      if (sqrt(tail) > thr) { k = i + 1; break; }
+    }
+    double d = 0.0;
+    for (int i = r - 1; i >= k; --i) d += a.sorted[i] * a.sorted[i];
+    SvdInfo inf;
+    inf.rank = k;
+    inf.overflow = k > a.max_rank ? 1 : 0;
+    inf.sweeps = converged ? sweeps + 1 : -1;
+    inf.pad = r;
+    inf.s0 = s0;
+    inf.discarded = sqrt(d);
+    *a.info = inf;
+    if (a.copy) *a.copy = inf;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+small_svd_kernel(SmallSvdArgs a) {
+  small_svd_body(a);
+}
+
+__global__ void __launch_bounds__(256)
+small_svd_batch_kernel(const SmallSvdArgs* probs) {
+  const SmallSvdArgs a = probs[blockIdx.x];
+  small_svd_body(a);
+}
+
 // Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR: one
 // workgroup per column, the column held in LDS, reflectors applied from
 // the last to the first; row i = a*B + b goes to dst[j*dj + a*da + b*db].

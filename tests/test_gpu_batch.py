@@ -67,3 +67,24 @@ def test_batch_rejects_unsupported():
     _, bath2, params2 = models.spin_boson(0.5, 5.0, 0.2, 0.05, 4, 1e-7)
     with pytest.raises(ValueError):
         oqupy.pt_tempo_compute_batch([bath2, bath2], 0.0, [0.5], params2)
+
+
+def test_dynamics_batch_many_systems_one_pt():
+    """compute_dynamics_batch equals compute_dynamics per system."""
+    _announce("dynamics batch: 5 systems, one process tensor")
+    _, bath, params = models.spin_boson(0.5, 5.0, 0.2, 0.05, 6, 1e-7)
+    n = 16
+    pt = oqupy.pt_tempo_compute(bath, 0.0, n * 0.05 + 1e-11, params)
+    systems = [oqupy.System(0.5 * om * op.sigma("x")) for om in (0.5, 1.0, 2.0)]
+    systems.append(oqupy.System(0.5 * op.sigma("x") + 0.3 * op.sigma("z")))
+    systems.append(oqupy.TimeDependentSystem(
+        lambda t: 0.5 * np.cos(2.0 * t) * op.sigma("x")))
+    rhos = [op.spin_dm("z+"), op.spin_dm("x+"), op.spin_dm("z-"),
+            op.spin_dm("y+"), op.spin_dm("z+")]
+    batch = oqupy.compute_dynamics_batch(systems, pt, rhos)
+    for system, rho0, dyn_b in zip(systems, rhos, batch):
+        dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                     initial_state=rho0)
+        assert np.abs(np.asarray(dyn_b.states) - np.asarray(dyn.states)).max() \
+            < 1e-13
+        assert np.allclose(dyn_b.times, dyn.times)
