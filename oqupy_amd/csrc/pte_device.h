@@ -41,13 +41,65 @@ __device__ __forceinline__ void cfmac(double2& acc, double2 a, double2 b) {
 }
 __device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
 
-// Sum over one 64-lane wavefront; every lane returns lane 0's total, so the
-// result is bit-identical across the wavefront.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return __shfl(v, 0, 64);
+// Cross-lane moves of a double by DPP (two 32-bit halves): lane i reads the
+// lane given by the control word (quad permutes, row mirrors).
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within 8
+constexpr int kDppMirror = 0x140;     // lane i <-> 15 - i within 16
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// Sums over aligned groups of G lanes (G a power of two <= 64) of four
+// values at once. Every step adds a partner's value symmetrically (x + y on
+// one lane, y + x on the other), so all lanes of a group end with the same
+// bits.
+__device__ __forceinline__ void group_sum4(double& a, double& b, double& c, double& d, int G) {
+  if (G >= 2) {
+    a += dpp_d<kDppXor1>(a); b += dpp_d<kDppXor1>(b);
+    c += dpp_d<kDppXor1>(c); d += dpp_d<kDppXor1>(d);
+  }
+  if (G >= 4) {
+    a += dpp_d<kDppXor2>(a); b += dpp_d<kDppXor2>(b);
+    c += dpp_d<kDppXor2>(c); d += dpp_d<kDppXor2>(d);
+  }
+  if (G >= 8) {
+    a += dpp_d<kDppHalfMirror>(a); b += dpp_d<kDppHalfMirror>(b);
+    c += dpp_d<kDppHalfMirror>(c); d += dpp_d<kDppHalfMirror>(d);
+  }
+  if (G >= 16) {
+    a += dpp_d<kDppMirror>(a); b += dpp_d<kDppMirror>(b);
+    c += dpp_d<kDppMirror>(c); d += dpp_d<kDppMirror>(d);
+  }
+  if (G >= 32) {
+    a += __shfl_xor(a, 16, 64); b += __shfl_xor(b, 16, 64);
+    c += __shfl_xor(c, 16, 64); d += __shfl_xor(d, 16, 64);
+  }
+  if (G >= 64) {
+    a += __shfl_xor(a, 32, 64); b += __shfl_xor(b, 32, 64);
+    c += __shfl_xor(c, 32, 64); d += __shfl_xor(d, 32, 64);
+  }
+}
+
+__device__ __forceinline__ double group_sum1(double a, int G) {
+  double b = 0.0, c = 0.0, d = 0.0;
+  group_sum4(a, b, c, d, G);
+  return a;
+}
+
+// Sum over one 64-lane wavefront; every lane gets the same bits.
+__device__ __forceinline__ double wave_sum(double v) { return group_sum1(v, 64); }
 
 // Sum over a 256-thread workgroup with one __syncthreads: wavefront sums go
 // to slot[parity][wave]; alternating the parity between calls keeps a fast

@@ -1108,10 +1108,7 @@ build_rh_batch_kernel(const RhArgs* pa) {
 // values, order and cut. Barriers are workgroup barriers only; column pairs
 // (and QR columns) are reduced by lane groups with wavefront shuffles.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double group_sum(double v, int G) {
-  for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return __shfl(v, (threadIdx.x & 63) & ~(G - 1), 64);
-}
+__device__ __forceinline__ double group_sum(double v, int G) { return group_sum1(v, G); }
 
 __device__ __forceinline__ int pow2_ceil(int x) {
   int p = 1;
@@ -1122,8 +1119,6 @@ __device__ __forceinline__ int pow2_ceil(int x) {
 __device__ void small_svd_body(const SmallSvdArgs& a) {
   extern __shared__ double2 lds_small[];
   __shared__ double slot[2][4];
-  __shared__ int s_piv;
-  __shared__ double s_total;
   int parity = 0;
   const int m = a.m, n = a.n, tid = threadIdx.x, lane = tid & 63;
   double2* As = lds_small;                                  // m x n, ld m
@@ -1132,11 +1127,14 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   int* done = reinterpret_cast<int*>(vn2 + n);
   const double eps = 2.220446049250313e-16;
 
+  PHASE_INIT();
   for (long e = tid; e < (long)m * n; e += 256) As[e] = a.A[e];
   __syncthreads();
+  PHASE(6);
   // ---- pivoted QR in LDS ----
   const int NP = pow2_ceil(n);
-  const int G = NP >= 4 ? 256 / NP : 64;         // threads per column (<= 64)
+  // threads per column: at most 64, at most 256 / NP, no more than the rows
+  const int G = min(min(NP >= 4 ? 256 / NP : 64, pow2_ceil(m)), 64);
   const int myc = tid / G, g = tid % G;
   {
     double part = 0.0;
@@ -1146,36 +1144,35 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     if (myc < n && g == 0) { vn1[myc] = nrm; vn2[myc] = nrm; done[myc] = 0; }
   }
   __syncthreads();
+  // pivot (largest residual norm, ties to the smaller column) and residual
+  // Frobenius norm squared, computed identically by every wavefront
+  int p;
+  double total;
   auto choose = [&]() {
-    // wavefront 0: pivot (largest residual norm, ties to the smaller column)
-    // and the residual Frobenius norm squared
-    if (tid < 64) {
-      double bv = -1.0, sv = 0.0;
-      int bc = -1;
-      for (int c = lane; c < n; c += 64)
-        if (!done[c]) { sv += vn1[c] * vn1[c]; better(bv, bc, vn1[c], c); }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off, 64);
-        const int oc = __shfl_xor(bc, off, 64);
-        better(bv, bc, ov, oc);
-      }
-      sv = wave_sum(sv);
-      if (lane == 0) { s_piv = bc; s_total = sv; }
-    }
-    __syncthreads();
+    double bv = -1.0, sv = 0.0;
+    int bc = -1;
+    for (int c = lane; c < n; c += 64)
+      if (!done[c]) { sv += vn1[c] * vn1[c]; better(bv, bc, vn1[c], c); }
+    better(bv, bc, dpp_d<kDppXor1>(bv), dpp_i<kDppXor1>(bc));
+    better(bv, bc, dpp_d<kDppXor2>(bv), dpp_i<kDppXor2>(bc));
+    better(bv, bc, dpp_d<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bc));
+    better(bv, bc, dpp_d<kDppMirror>(bv), dpp_i<kDppMirror>(bc));
+    better(bv, bc, __shfl_xor(bv, 16, 64), __shfl_xor(bc, 16, 64));
+    better(bv, bc, __shfl_xor(bv, 32, 64), __shfl_xor(bc, 32, 64));
+    p = bc;
+    total = wave_sum(sv);
   };
   choose();
-  const double drop2 = eps * eps * s_total;
+  const double drop2 = eps * eps * total;
   const double tol3z = 1.4901161193847656e-08;
   const int kmax = m < n ? m : n;
   int j = 0;
-  while (j < kmax && s_piv >= 0 && s_total > drop2) {
-    const int p = s_piv;
+  while (j < kmax && p >= 0 && total > drop2) {
     const double2* pc = As + (long)p * m;
+    // every wavefront sums the pivot column itself (no workgroup barrier)
     double part = 0.0;
-    for (int i = j + 1 + tid; i < m; i += 256) part += cabs2(pc[i]);
-    const double xnorm2 = block_sum256(part, slot, parity);
+    for (int i = j + 1 + lane; i < m; i += 64) part += cabs2(pc[i]);
+    const double xnorm2 = wave_sum(part);
     const double2 alpha = pc[j];
     double2 tau, scale;
     double beta;
@@ -1195,43 +1192,48 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
       a.tau[j] = tau;
       a.vscale[j] = scale;
       a.beta[j] = beta;
-      done[p] = 1;
     }
-    __syncthreads();
     const bool reflect = (tau.x != 0.0 || tau.y != 0.0);
+    // the group of column myc applies H_j^H and downdates the column norm;
+    // only that group touches the column, its norms and its done flag
     if (myc < n && !done[myc]) {
-      double2* col = As + (long)myc * m;
-      if (reflect) {
-        double2 w = make_double2(0.0, 0.0);
-        for (int i = j + g; i < m; i += G) {
-          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
-          cfmac(w, vi, col[i]);
+      if (myc == p) {
+        if (g == 0) done[myc] = 1;
+      } else {
+        double2* col = As + (long)myc * m;
+        double2 rj2 = col[j];
+        if (reflect) {
+          double2 w = make_double2(0.0, 0.0);
+          for (int i = j + g; i < m; i += G) {
+            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+            cfmac(w, vi, col[i]);
+          }
+          {
+            double z0 = 0.0, z1 = 0.0;
+            group_sum4(w.x, w.y, z0, z1, G);
+          }
+          const double2 tw = cmul(make_double2(tau.x, -tau.y), w);
+          for (int i = j + g; i < m; i += G) {
+            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+            col[i] = csub(col[i], cmul(vi, tw));
+          }
+          // row j of the column after the reflection (held by lane g = 0)
+          rj2 = csub(rj2, tw);
         }
-        w.x = group_sum(w.x, G);
-        w.y = group_sum(w.y, G);
-        const double2 tw = cmul(make_double2(tau.x, -tau.y), w);
-        for (int i = j + g; i < m; i += G) {
-          const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
-          col[i] = csub(col[i], cmul(vi, tw));
-        }
-      }
-    }
-    __syncthreads();
-    if (myc < n && !done[myc]) {
-      const double2* col = As + (long)myc * m;
-      double nv = vn1[myc];
-      if (nv != 0.0) {
-        const double rj = sqrt(cabs2(col[j]));
-        double temp = 1.0 - (rj / nv) * (rj / nv);
-        temp = temp > 0.0 ? temp : 0.0;
-        const double ratio = nv / vn2[myc];
-        if (temp * ratio * ratio <= tol3z) {
-          double pp = 0.0;
-          for (int i = j + 1 + g; i < m; i += G) pp += cabs2(col[i]);
-          nv = (j + 1 < m) ? sqrt(group_sum(pp, G)) : 0.0;
-          if (g == 0) { vn1[myc] = nv; vn2[myc] = nv; }
-        } else if (g == 0) {
-          vn1[myc] = nv * sqrt(temp);
+        double nv = vn1[myc];
+        if (nv != 0.0) {
+          const double rj = sqrt(cabs2(rj2));
+          double temp = 1.0 - (rj / nv) * (rj / nv);
+          temp = temp > 0.0 ? temp : 0.0;
+          const double ratio = nv / vn2[myc];
+          if (temp * ratio * ratio <= tol3z) {
+            double pp = 0.0;
+            for (int i = j + 1 + g; i < m; i += G) pp += cabs2(col[i]);
+            nv = (j + 1 < m) ? sqrt(group_sum(pp, G)) : 0.0;
+            if (g == 0) { vn1[myc] = nv; vn2[myc] = nv; }
+          } else if (g == 0) {
+            vn1[myc] = nv * sqrt(temp);
+          }
         }
       }
     }
@@ -1240,6 +1242,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     ++j;
   }
   const int r = j;
+  PHASE(13);
   if (tid == 0) {
     int t = r;
     for (int c = 0; c < n; ++c)
@@ -1282,7 +1285,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   const int re = r + (r & 1);
   const int P = re / 2;
   const int PG = pow2_ceil(P);
-  const int GJ = PG >= 4 ? 256 / PG : 64;         // threads per pair (<= 64)
+  const int GJ = min(min(PG >= 4 ? 256 / PG : 64, pow2_ceil(n)), 64);   // threads per pair
   const int q = tid / GJ, gj = tid % GJ;
   int sweeps = 0;
   bool converged = false;
@@ -1305,10 +1308,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
           be += cabs2(yj[i]);
           cfmac(ga, yi[i], yj[i]);
         }
-        al = group_sum(al, GJ);
-        be = group_sum(be, GJ);
-        ga.x = group_sum(ga.x, GJ);
-        ga.y = group_sum(ga.y, GJ);
+        group_sum4(al, be, ga.x, ga.y, GJ);
         double rot[4];
         if (jacobi_rotation(al, be, ga, tol2, abs_tol2, rot)) {
           any = 1;
@@ -1337,32 +1337,42 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
       break;
     }
   }
+  PHASE(14);
   // ---- singular values, order, cut; Z and W to global memory ----
   for (long e = tid; e < (long)n * r; e += 256) a.Y[e] = Ys[e];
   for (long e = tid; e < (long)r * r; e += 256) a.W[e] = Wsm[e];
+  __shared__ double s_sig[kSmallMaxCols], s_sorted[kSmallMaxCols];
   for (int c = tid / 64; c < r; c += 4) {
     double part2 = 0.0;
     for (int i = lane; i < n; i += 64) part2 += cabs2(Ys[i + (long)c * n]);
     const double sg = sqrt(wave_sum(part2));
-    if (lane == 0) a.sigma[c] = sg;
+    if (lane == 0) { s_sig[c] = sg; a.sigma[c] = sg; }
   }
-  __threadfence_block();
   __syncthreads();
-  if (tid < r) rank_sort_body(a.sigma, r, a.perm, a.sorted, tid);
-  __threadfence_block();
+  if (tid < r) {
+    const double sj = s_sig[tid];
+    int rank = 0;
+    for (int i = 0; i < r; ++i) {
+      const double si = s_sig[i];
+      rank += (si > sj) || (si == sj && i < tid);
+    }
+    a.perm[rank] = tid;
+    a.sorted[rank] = sj;
+    s_sorted[rank] = sj;
+  }
   __syncthreads();
   if (tid == 0) {
     // smallest k with sqrt(sum_{i>=k} s_i^2) <= epsrel s_0 (suffix sums)
-    const double s0 = a.sorted[0];
+    const double s0 = s_sorted[0];
     const double thr = a.epsrel * s0;
     double tail = 0.0;
     int k = 0;
     for (int i = r - 1; i >= 0; --i) {
-      tail += a.sorted[i] * a.sorted[i];
+      tail += s_sorted[i] * s_sorted[i];
       if (sqrt(tail) > thr) { k = i + 1; break; }
     }
     double d = 0.0;
-    for (int i = r - 1; i >= k; --i) d += a.sorted[i] * a.sorted[i];
+    for (int i = r - 1; i >= k; --i) d += s_sorted[i] * s_sorted[i];
     SvdInfo inf;
     inf.rank = k;
     inf.overflow = k > a.max_rank ? 1 : 0;
@@ -1373,6 +1383,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     *a.info = inf;
     if (a.copy) *a.copy = inf;
   }
+  PHASE(15);
 }
 
 __global__ void __launch_bounds__(256)
