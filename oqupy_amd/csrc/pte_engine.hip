@@ -221,8 +221,8 @@ struct Workspace {
   // accumulated timing of the Jacobi SVD kernel (HIP events on this stream)
   hipEvent_t evj0 = nullptr, evj1 = nullptr;
   hipEvent_t evq0 = nullptr, evq1 = nullptr;
-  long jac_count = 0, qr_count = 0;
-  double qr_ms = 0.0;
+  long jac_count = 0, qr_count = 0, small_count = 0;
+  double qr_ms = 0.0, small_ms = 0.0;
   // optional per-SVD log (environment variable PTE_SVD_TRACE = file path):
   // m, n, pivoted-QR rank (0 without QR), kept rank, sweeps, QR ms, Jacobi ms
   FILE* trace = nullptr;
@@ -387,7 +387,9 @@ struct Workspace {
   }
 
   // after synchronisation: checks, statistics, kept rank
-  int jacobi_check(int m, int n, float ms) {
+  // `small`: the SVD ran in small_svd_kernel (its time is kept apart from
+  // the Jacobi kernel's statistics)
+  int jacobi_check(int m, int n, float ms, bool small = false) {
     require(info_h->sweeps > 0, PTE_ERR_NUMERIC,
             "Jacobi SVD did not converge (" + std::to_string(m) + "x" +
                 std::to_string(n) + ")");
@@ -399,10 +401,15 @@ struct Workspace {
     // of Golub & Van Loan, times 4 for complex arithmetic) and the HBM bytes
     // of reading and writing M once plus writing V
     const double mm = std::max(m, n), nn = std::min(m, n);
-    jac_ms += ms;
-    jac_count += 1;
-    jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
-    jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
+    if (small) {
+      small_ms += ms;
+      small_count += 1;
+    } else {
+      jac_ms += ms;
+      jac_count += 1;
+      jac_flops += 4.0 * (6.0 * mm * nn * nn + 20.0 * nn * nn * nn);
+      jac_bytes += 16.0 * (2.0 * (double)m * n + (double)n * n);
+    }
     if (trace) {
       if (tr_r == 0) { tr_m = n; tr_n = m; tr_qr_ms = 0.f; }
       std::fprintf(trace, "%d,%d,%d,%d,%d,%.4f,%.4f\n", tr_m, tr_n, tr_r, info_h->rank,
@@ -514,7 +521,7 @@ struct Workspace {
     qr_count += 1;
     tr_m = m; tr_n = n; tr_r = r; tr_qr_ms = 0.f;
     qr_m = m; qr_n = n; qr_r = r;
-    return jacobi_check(n, r, ms);
+    return jacobi_check(n, r, ms, true);
   }
 
   int svd_small(int m, int n) {
