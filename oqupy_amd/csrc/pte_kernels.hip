@@ -336,6 +336,21 @@ __device__ __forceinline__ bool needs_rotation(double app, double aqq, double ma
          mag2 > 0.0;
 }
 
+// 1/sqrt(x) and 1/x for normal positive x: the hardware estimates refined
+// by Newton steps (quadratic convergence) to full double precision.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  y = y * fma(-0.5 * x * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = y * fma(-x, y, 2.0);
+  y = y * fma(-x, y, 2.0);
+  return y;
+}
+
 // Rotation of that block: J = [[cs, sn], [-sn conj(ph), cs conj(ph)]],
 // ph = apq / |apq|, chosen so that J^H [[app, apq], [conj(apq), aqq]] J is
 // diagonal (Rutishauser's formulas). r = {cs, sn, Re ph, Im ph}.
@@ -344,11 +359,19 @@ __device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 
   const double mag2 = cabs2(apq);
   r[0] = 1.0; r[1] = 0.0; r[2] = 1.0; r[3] = 0.0;
   if (!needs_rotation(app, aqq, mag2, tol2, abs_tol2)) return false;
-  const double inv = rsqrt(mag2);
+  const double inv = rsqrt_nr(mag2);
   const double zeta = (aqq - app) * 0.5 * inv;
-  const double t = (zeta == 0.0)
-      ? 1.0 : copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double cs = rsqrt(1.0 + t * t);
+  double t;
+  if (zeta == 0.0) {
+    t = 1.0;
+  } else if (fabs(zeta) > 1e150) {
+    t = copysign(0.5, zeta) * rcp_nr(fabs(zeta));
+  } else {
+    const double q = fma(zeta, zeta, 1.0);
+    const double root = q * rsqrt_nr(q);             // sqrt(1 + zeta^2)
+    t = copysign(1.0, zeta) * rcp_nr(fabs(zeta) + root);
+  }
+  const double cs = rsqrt_nr(fma(t, t, 1.0));
   r[0] = cs; r[1] = cs * t; r[2] = apq.x * inv; r[3] = apq.y * inv;
   return true;
 }
@@ -1119,6 +1142,9 @@ __device__ __forceinline__ int pow2_ceil(int x) {
 __device__ void small_svd_body(const SmallSvdArgs& a) {
   extern __shared__ double2 lds_small[];
   __shared__ double slot[2][4];
+  __shared__ int q_piv[kSmallMaxCols];
+  __shared__ double2 q_tau[kSmallMaxCols], q_vscale[kSmallMaxCols];
+  __shared__ double q_beta[kSmallMaxCols];
   int parity = 0;
   const int m = a.m, n = a.n, tid = threadIdx.x, lane = tid & 63;
   double2* As = lds_small;                                  // m x n, ld m
@@ -1188,10 +1214,12 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
       tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
     }
     if (tid == 0) {
-      a.piv[j] = p;
-      a.tau[j] = tau;
-      a.vscale[j] = scale;
-      a.beta[j] = beta;
+      // kept in LDS until the end: a global store here would be waited for
+      // at every barrier
+      q_piv[j] = p;
+      q_tau[j] = tau;
+      q_vscale[j] = scale;
+      q_beta[j] = beta;
     }
     const bool reflect = (tau.x != 0.0 || tau.y != 0.0);
     // the group of column myc applies H_j^H and downdates the column norm;
@@ -1204,6 +1232,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
         double2 rj2 = col[j];
         if (reflect) {
           double2 w = make_double2(0.0, 0.0);
+#pragma unroll 4
           for (int i = j + g; i < m; i += G) {
             const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
             cfmac(w, vi, col[i]);
@@ -1213,6 +1242,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
             group_sum4(w.x, w.y, z0, z1, G);
           }
           const double2 tw = cmul(make_double2(tau.x, -tau.y), w);
+#pragma unroll 4
           for (int i = j + g; i < m; i += G) {
             const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
             col[i] = csub(col[i], cmul(vi, tw));
@@ -1246,8 +1276,17 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   if (tid == 0) {
     int t = r;
     for (int c = 0; c < n; ++c)
-      if (!done[c]) a.piv[t++] = c;
+      if (!done[c]) q_piv[t++] = c;
     *a.qrank = r;
+  }
+  __syncthreads();
+  for (int t = tid; t < n; t += 256) {
+    a.piv[t] = q_piv[t];
+    if (t < r) {
+      a.tau[t] = q_tau[t];
+      a.vscale[t] = q_vscale[t];
+      a.beta[t] = q_beta[t];
+    }
   }
   // reflectors and R back to global memory (apply_q_kernel reads them)
   for (long e = tid; e < (long)m * n; e += 256) a.A[e] = As[e];
@@ -1267,10 +1306,10 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   double2* Wsm = Ys + (long)n * r;
   for (long e = tid; e < (long)n * r; e += 256) {
     const int sidx = e % n, t = e / n;
-    const int ps = a.piv[sidx];
+    const int ps = q_piv[sidx];
     double2 v = make_double2(0.0, 0.0);
     if (t < sidx) v = cconj(a.A[t + (long)ps * m]);   // As is being overwritten
-    else if (t == sidx) v = make_double2(a.beta[sidx], 0.0);
+    else if (t == sidx) v = make_double2(q_beta[sidx], 0.0);
     Ys[e] = v;
   }
   __syncthreads();
@@ -1294,7 +1333,17 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     for (int round = 0; round < (re > 1 ? re - 1 : 1); ++round) {
       int ci = -1, cj = -1;
       if (q < P && re > 1) {
-        tournament_pair(re, round, q, ci, cj);
+        // round-robin pairing of tournament_pair without integer division
+        const int L = re - 1;
+        if (q == 0) {
+          ci = L;
+          cj = round;
+        } else {
+          ci = round + q;
+          if (ci >= L) ci -= L;
+          cj = round - q;
+          if (cj < 0) cj += L;
+        }
         if (ci > cj) { const int tt = ci; ci = cj; cj = tt; }
         if (cj >= r) { ci = -1; cj = -1; }
       }
@@ -1303,6 +1352,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
         double2* yj = Ys + (long)cj * n;
         double al = 0.0, be = 0.0;
         double2 ga = make_double2(0.0, 0.0);
+#pragma unroll 4
         for (int i = gj; i < n; i += GJ) {
           al += cabs2(yi[i]);
           be += cabs2(yj[i]);
@@ -1314,6 +1364,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
           any = 1;
           const double c = rot[0], sn = rot[1];
           const double2 cph = make_double2(rot[2], -rot[3]);
+#pragma unroll 4
           for (int i = gj; i < n; i += GJ) {
             const double2 xp = yi[i];
             const double2 xq = cmul(yj[i], cph);
