@@ -8,6 +8,7 @@ correlations, systems and influence tensors are host-side FP64 code.
 
 from oqupy_amd import operators
 from oqupy_amd.bath import Bath
+from oqupy_amd.control import Control
 from oqupy_amd.bath_correlations import (CustomCorrelations, CustomSD,
                                          PowerLawSD)
 from oqupy_amd.dynamics import Dynamics
@@ -21,7 +22,7 @@ from oqupy_amd.tempo import (Tempo, TempoParameters, influence_matrix,
 from oqupy_amd.version import __version__
 
 __all__ = [
-    'Bath', 'CustomCorrelations', 'CustomSD', 'Dynamics', 'PowerLawSD',
+    'Bath', 'Control', 'CustomCorrelations', 'CustomSD', 'Dynamics', 'PowerLawSD',
     'PtTempo', 'SimpleProcessTensor', 'System', 'Tempo', 'TempoParameters',
     'TimeDependentSystem', 'TrivialProcessTensor', 'compute_dynamics',
     'compute_dynamics_batch',

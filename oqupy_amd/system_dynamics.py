@@ -1,8 +1,12 @@
 """``compute_dynamics``: system dynamics through a process tensor.
 
 Restates ``compute_dynamics`` and its input parsing from
-``oqupy/system_dynamics.py:41-182, 478-559`` for one (or no) process tensor
-and no controls. The per-step contraction -- first half-step propagator,
+``oqupy/system_dynamics.py:41-182, 478-559`` for one (or no) process tensor.
+Control operations are folded into the half-step propagators on the host
+(the post-measurement control of step k into the first half-step of step k,
+the pre-measurement control of step k+1 into the second half-step of step k,
+that of step 0 into the initial state), which reproduces the reference's
+order of application. The per-step contraction -- first half-step propagator,
 PT-MPO tensor (with the basis transforms of a non-diagonal coupling), second
 half-step, and the cap readout of every state -- runs in the MI355X engine
 (``pte_dynamics_run`` / ``pte_dynamics_run_engine``).
@@ -15,6 +19,7 @@ from numpy import ndarray
 
 from oqupy_amd.backends import engine as eng
 from oqupy_amd.config import INTEGRATE_EPSREL, SUBDIV_LIMIT
+from oqupy_amd.control import Control
 from oqupy_amd.dynamics import Dynamics
 from oqupy_amd.process_tensor import BaseProcessTensor, TrivialProcessTensor
 from oqupy_amd.system import System, TimeDependentSystem
@@ -33,9 +38,9 @@ def _compute_dynamics_input_parse(system, initial_state, dt, num_steps,
                      "process_tensor")
     process_tensors = [process_tensor] \
         if isinstance(process_tensor, BaseProcessTensor) else process_tensor
-    if control is not None:
-        raise NotImplementedError("Controls are not supported by this "
-                                  "package's compute_dynamics.")
+    if control is None:
+        control = Control(hs_dim)
+    check_isinstance(control, Control, "control")
     start_time = check_convert(start_time, float, "start_time")
     if initial_state is None:
         raise ValueError("An initial state must be specified.")
@@ -74,7 +79,25 @@ def _compute_dynamics_input_parse(system, initial_state, dt, num_steps,
                    "infinite.")
         num_steps = int(max_step)
     return (system, initial_state, dt, num_steps, start_time, process_tensors,
-            record_all, hs_dim)
+            control, record_all, hs_dim)
+
+
+def _fold_controls(control: Control, props: ndarray, rho0: ndarray,
+                   num_steps: int, dt: float, start_time: float) -> ndarray:
+    """Fold the pre/post-measurement controls into the half-step
+    propagators (in place) and the initial state (returned)."""
+    pre, _ = control.get_controls(0, dt=dt, start_time=start_time)
+    if pre is not None:
+        rho0 = np.asarray(pre, dtype=np.complex128) @ rho0
+    for step in range(num_steps):
+        _, post = control.get_controls(step, dt=dt, start_time=start_time)
+        pre_next, _ = control.get_controls(step + 1, dt=dt,
+                                           start_time=start_time)
+        if post is not None:
+            props[step, 0] = props[step, 0] @ post
+        if pre_next is not None:
+            props[step, 1] = pre_next @ props[step, 1]
+    return rho0
 
 
 def compute_dynamics(
@@ -94,7 +117,7 @@ def compute_dynamics(
     (oqupy/system_dynamics.py:41-182). States at ``start_time + k dt`` for
     ``k = 0 .. num_steps`` (only the last one if ``record_all`` is False)."""
     (system, initial_state, dt, num_steps, start_time, process_tensors,
-     record_all, hs_dim) = _compute_dynamics_input_parse(
+     control, record_all, hs_dim) = _compute_dynamics_input_parse(
          system, initial_state, dt, num_steps, start_time, process_tensor,
          control, record_all)
     process_tensors = [pt for pt in process_tensors
@@ -111,6 +134,8 @@ def compute_dynamics(
         props[step, 0] = first
         props[step, 1] = second
     rho0 = eng.as_c128(initial_state.reshape(d2))
+    rho0 = eng.as_c128(_fold_controls(control, props, rho0, num_steps, dt,
+                                      start_time))
     states = np.empty((num_steps + 1, d2), dtype=np.complex128)
     eng.require_gpu()
     lib = eng.load_library()
@@ -179,7 +204,8 @@ def compute_dynamics_batch(
         start_time: Optional[float] = 0.0,
         record_all: Optional[bool] = True,
         subdiv_limit: Optional[int] = SUBDIV_LIMIT,
-        liouvillian_epsrel: Optional[float] = INTEGRATE_EPSREL
+        liouvillian_epsrel: Optional[float] = INTEGRATE_EPSREL,
+        control_list: Optional[List[Control]] = None
         ) -> List[Dynamics]:
     """``compute_dynamics`` of many systems through one process tensor built by
     the engine (the "many drives, one environment" use of a process tensor):
@@ -198,6 +224,10 @@ def compute_dynamics_batch(
         else [initial_state] * len(systems)
     if len(states0) != len(systems):
         raise ValueError("`initial_state` needs one entry per system.")
+    if control_list is None:
+        control_list = [None] * len(systems)
+    if len(control_list) != len(systems):
+        raise ValueError("`control_list` needs one entry per system.")
     num_steps = len(process_tensor)
     dt = process_tensor.dt
     hs_dim = process_tensor.hilbert_space_dimension
@@ -205,17 +235,22 @@ def compute_dynamics_batch(
     props = np.empty((len(systems), num_steps, 2, d2, d2),
                      dtype=np.complex128)
     rho0 = np.empty((len(systems), d2), dtype=np.complex128)
-    for s, (system, st) in enumerate(zip(systems, states0)):
+    for s, (system, st, ctrl) in enumerate(zip(systems, states0,
+                                               control_list)):
         # the same argument checks as compute_dynamics
-        _compute_dynamics_input_parse(system, st, dt, num_steps, start_time,
-                                      process_tensor, None, record_all)
+        ctrl = _compute_dynamics_input_parse(
+            system, st, dt, num_steps, start_time, process_tensor, ctrl,
+            record_all)[6]
         propagators = system.get_propagators(dt, float(start_time),
                                              subdiv_limit, liouvillian_epsrel)
         for step in range(num_steps):
             first, second = propagators(step)
             props[s, step, 0] = first
             props[s, step, 1] = second
-        rho0[s] = np.asarray(st, dtype=np.complex128).reshape(d2)
+        rho0[s] = _fold_controls(ctrl, props[s],
+                                 np.asarray(st, dtype=np.complex128
+                                            ).reshape(d2),
+                                 num_steps, dt, float(start_time))
     tin = None if process_tensor.transform_in is None \
         else eng.as_c128(process_tensor.transform_in)
     tout = None if process_tensor.transform_out is None \

@@ -1,12 +1,15 @@
 """``compute_dynamics`` with process tensors (FP64 oracle).
 
-Restates ``oqupy/system_dynamics.py:41-182`` without controls: per step the
-state (with one bond leg per process tensor) is recorded through the caps
-(``_apply_caps``, 643-651), propagated by the first half-step super-operator
+Restates ``oqupy/system_dynamics.py:41-182``: per step the pre-measurement
+control is applied (``controls(step)``, lines 131-138), the state (with one
+bond leg per process tensor) is recorded through the caps
+(``_apply_caps``, 643-651), the post-measurement control is applied (151-154),
+then the state is propagated by the first half-step super-operator
 (``_apply_system_superoperator``, 631-640), by each PT-MPO tensor
 (``_apply_pt_mpos``, 654-700, with the rank-3 -> rank-4 delta of
 ``oqupy/process_tensor.py:345-355`` and its optional basis transforms) and by
-the second half-step.
+the second half-step; after the last step only the pre-measurement control
+of step N is applied before the final readout.
 
 Pinning: golden states of the reference physics tests, the exact dephasing
 formula, and the two-environments-at-alpha equals one-at-2-alpha invariant.
@@ -38,9 +41,12 @@ def compute_dynamics(propagators: Callable[[int], tuple],
                      initial_state: np.ndarray, num_steps: int,
                      pt_tensors: Sequence[List[np.ndarray]],
                      pt_caps: Sequence[List[np.ndarray]],
-                     transforms: Optional[Sequence[tuple]] = None
+                     transforms: Optional[Sequence[tuple]] = None,
+                     controls: Optional[Callable[[int], tuple]] = None
                      ) -> List[np.ndarray]:
-    """States ``rho(t_0) ... rho(t_N)`` (as d x d matrices)."""
+    """States ``rho(t_0) ... rho(t_N)`` (as d x d matrices). ``controls(step)``
+    returns the (pre, post) measurement super-operators of ``step`` (None
+    for none), as ``Control.get_controls`` (oqupy/control.py)."""
     d = initial_state.shape[0]
     n_env = len(pt_tensors)
     if transforms is None:
@@ -55,8 +61,16 @@ def compute_dynamics(propagators: Callable[[int], tuple],
             vec = np.tensordot(pt_caps[env][step], vec, axes=([0], [0]))
         return vec.reshape(d, d)
 
+    def apply(state, sup_op):
+        if sup_op is None:
+            return state
+        return np.tensordot(state, np.asarray(sup_op).T, axes=([-1], [0]))
+
     for step in range(num_steps):
+        pre, post = controls(step) if controls is not None else (None, None)
+        state = apply(state, pre)
         states.append(readout(state, step))
+        state = apply(state, post)
         first, second = propagators(step)
         state = np.tensordot(state, first.T, axes=([-1], [0]))
         for env in range(n_env):
@@ -70,5 +84,7 @@ def compute_dynamics(propagators: Callable[[int], tuple],
             new = np.moveaxis(new, -2, 0)                  # [b, rest..., out]
             state = np.moveaxis(new, 0, env)
         state = np.tensordot(state, second.T, axes=([-1], [0]))
+    if controls is not None:
+        state = apply(state, controls(num_steps)[0])
     states.append(readout(state, num_steps))
     return states

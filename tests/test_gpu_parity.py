@@ -255,3 +255,43 @@ def test_engine_errors_are_reported():
     with pytest.raises(RuntimeError):
         bad = np.ones((4, 4)) * np.nan
         engine.split_svd(bad, 1e-7)
+
+
+def test_compute_dynamics_controls():
+    """Pre/post-measurement controls (by step and by time, at the first and
+    the last step) folded into the engine's propagators vs the oracle's
+    step-by-step application."""
+    _announce("dynamics with controls")
+    from oracle.dynamics import compute_dynamics
+    from oracle.pt_tempo import PtTempoOracle, compute_caps
+    system, bath, params = models.spin_boson(0.5, 5.0, 0.2, 0.05, 6, 1e-7)
+    n = 16
+    pt = oqupy.pt_tempo_compute(bath, 0.0, n * 0.05 + 1e-11, params)
+    x = op.sigma("x")
+    pulse = op.left_right_super(x, x.conj().T)
+    control = oqupy.Control(2)
+    control.add_single(0, 0.5 * (np.eye(4) + pulse))
+    control.add_single(3, pulse)
+    control.add_single(7, pulse, post=True)
+    control.add_single(0.5, pulse)                     # step 10
+    control.add_single(n, pulse)                       # before the last readout
+    rho0 = op.spin_dm("z+")
+    dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                 initial_state=rho0, control=control)
+    infl = models.CachedInfluence(bath, params)
+    ref = PtTempoOracle(2, infl, np.ones(4), n, 6, 1e-7)
+    ref.compute()
+    tens = ref.mpo_tensors()
+    caps = compute_caps(tens, 2)
+    props = system.get_propagators(0.05, 0.0, None, None)
+    states = compute_dynamics(props, rho0, n, [tens], [caps],
+                              controls=lambda s: control.get_controls(s, dt=0.05))
+    assert np.abs(dyn.states - np.array(states)).max() < STATE_TOL
+    batch = oqupy.compute_dynamics_batch([system, system], pt, [rho0, rho0],
+                                         control_list=[control, None])
+    plain = oqupy.compute_dynamics(system=system, process_tensor=pt,
+                                   initial_state=rho0)
+    assert np.abs(np.asarray(batch[0].states) - np.asarray(dyn.states)).max() \
+        < 1e-13
+    assert np.abs(np.asarray(batch[1].states) - np.asarray(plain.states)).max() \
+        < 1e-13

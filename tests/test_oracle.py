@@ -189,3 +189,54 @@ def test_zip_and_sweep_match_dense_contraction():
     chain.sweep_left(mps, 2, 0, 1e-15)
     chain.sweep_right(mps, 0, 2, 1e-15)
     assert np.allclose(np.einsum('apb,bqc,crd->pqr', *mps), ref)
+
+
+# ---- controls (oqupy/control.py, system_dynamics.py:131-154) ----
+
+def test_control_bookkeeping():
+    from oqupy_amd.control import Control
+    c = Control(2)
+    a, b = 2.0 * np.eye(4), 3.0 * np.diag([1.0, 2.0, 3.0, 4.0])
+    c.add_single(3, a)
+    c.add_single(3, b)                    # composed: b @ a
+    c.add_single(0.1, 5.0 * np.eye(4), post=True)   # time 0.1 -> step 2
+    pre, post = c.get_controls(3, dt=0.05)
+    assert np.allclose(pre, b @ a) and post is None
+    pre, post = c.get_controls(2, dt=0.05)
+    assert pre is None and np.allclose(post, 5.0 * np.eye(4))
+    assert c.get_controls(2, dt=0.05, start_time=0.05) == (None, None)
+    with pytest.raises(TypeError):
+        c.add_single("3", a)
+    with pytest.raises(NotImplementedError):
+        c.add_continuous(lambda rho, t: rho)
+
+
+@pytest.mark.parametrize("post", [False, True])
+def test_oracle_controls_closed_form(post):
+    """A pi pulse about x on a precessing spin (no environment): the oracle's
+    placement of pre/post-measurement controls against exact unitary
+    evolution."""
+    import scipy.linalg as sl
+    from oracle.dynamics import compute_dynamics
+    from oqupy_amd import operators as op
+    from oqupy_amd.control import Control
+    from oqupy_amd.system import System
+    dt, n, k = 0.1, 8, 3
+    h = 0.7 * op.sigma("z") + 0.2 * op.sigma("x")
+    system = System(h)
+    props = system.get_propagators(dt, 0.0, None, None)
+    x = op.sigma("x")
+    pulse = op.left_right_super(x, x.conj().T)
+    control = Control(2)
+    control.add_single(k, pulse, post=post)
+    rho0 = op.spin_dm("x+")
+    states = compute_dynamics(props, rho0, n, [], [],
+                              controls=lambda s: control.get_controls(s, dt=dt))
+    u = lambda t: sl.expm(-1j * h * t)
+    for j in range(n + 1):
+        if j < k or (post and j == k):
+            ref = u(j * dt) @ rho0 @ u(j * dt).conj().T
+        else:
+            mid = x @ u(k * dt) @ rho0 @ u(k * dt).conj().T @ x
+            ref = u((j - k) * dt) @ mid @ u((j - k) * dt).conj().T
+        assert np.abs(states[j] - ref).max() < 1e-12, (j, post)
