@@ -219,6 +219,26 @@ int pte_dynamics_run_engine_batch(pte_pt_engine* e, int nsys, const double* prop
                                   const double* transform_out, const double* rho0,
                                   double* states_out);
 
+/* compute_dynamics of one system coupled to `count` (1..8) environments,
+ * each given by a completed, device-resident process tensor of the same
+ * length and dimension; transforms_in / transforms_out hold one pointer per
+ * environment (NULL entries for none). Replaces the loop of compute_dynamics
+ * over a list of process tensors (oqupy/system_dynamics.py:119-171, 654-700). */
+int pte_dynamics_run_engines(pte_pt_engine* const* engines, int count, const double* props,
+                             const double* const* transforms_in,
+                             const double* const* transforms_out, const double* rho0,
+                             double* states_out);
+
+/* The same for process tensors held on the host (read from files or built
+ * outside the engine): bonds is count x (num_steps + 1); tensors and caps are
+ * the environments' [a][b][i] sites and caps concatenated in order, with the
+ * layout of pte_dynamics_run (oqupy/system_dynamics.py:119-171). */
+int pte_dynamics_run_multi(int d2, int num_steps, int count, const int* bonds,
+                           const double* tensors, const double* caps, const double* props,
+                           const double* const* transforms_in,
+                           const double* const* transforms_out, const double* rho0,
+                           double* states_out);
+
 /* Truncated SVD of one matrix by the engine's kernels: M (m x n, row-major)
  * = U diag(S) Vh with the kept rank k = #{j : sqrt(sum_{i>=j} s_i^2) >
  * epsrel * s_0}; U is m x min(m,n) capacity (row-major m x k on return),

@@ -108,6 +108,12 @@ _SIGNATURES = {
     "pte_pt_batch_destroy": (None, [_P]),
     "pte_dynamics_run_engine_batch": (ctypes.c_int, [_P, ctypes.c_int, _DP, _DP,
                                                      _DP, _DP, _DP]),
+    "pte_dynamics_run_engines": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, _DP,
+                                                ctypes.POINTER(_DP),
+                                                ctypes.POINTER(_DP), _DP, _DP]),
+    "pte_dynamics_run_multi": (ctypes.c_int, [ctypes.c_int] * 3 + [_IP, _DP, _DP, _DP,
+                                              ctypes.POINTER(_DP),
+                                              ctypes.POINTER(_DP), _DP, _DP]),
 }
 
 PRECISIONS = {"c128": 0, "c64": 1}

@@ -1945,6 +1945,190 @@ dyn_readout_kernel(const double2* v, const double2* cap, int A, int d2, long vst
   }
 }
 
+// ---- one system coupled to several environments ----
+// The state carries one bond leg per process tensor: v[b_0 .. b_{E-1}][i],
+// flattened with the last environment fastest and the Liouville index i
+// innermost.
+
+constexpr int kMaxEnvironments = 8;
+
+// w[r][o] = sum_i M[o][i] v[r][i]  (M = P, or M = T^T when trans: w = v T)
+__global__ void __launch_bounds__(256)
+dyn_rows_kernel(const double2* v, long rows, int d2, const double2* M, int trans, double2* w) {
+  const long total = rows * d2;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const int o = e % d2;
+    const long r = e / d2;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < d2; ++i)
+      cfma(acc, trans ? M[i * d2 + o] : M[o * d2 + i], v[r * d2 + i]);
+    w[e] = acc;
+  }
+}
+
+// Environment e with the legs before it already advanced:
+//   out[l][b][q][o] = scale * sum_i (sum_a v[l][a][q][i] T[a][i][b]) Tout[i][o]
+// (Tout = identity when null); one thread per (l, b, q), T read in its
+// [a][i][b] layout.
+__global__ void __launch_bounds__(256)
+dyn_env_kernel(const double2* v, long L, int A, long Q, int d2, const double2* T, int B,
+               const double2* tout, double scale, double2* out) {
+  const long total = L * B * Q;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long q = e % Q;
+    const long t = e / Q;
+    const int b = t % B;
+    const long l = t / B;
+    double2 acc[16];
+    for (int i = 0; i < d2; ++i) acc[i] = make_double2(0.0, 0.0);
+    for (int a = 0; a < A; ++a) {
+      const double2* vr = v + ((l * A + a) * Q + q) * d2;
+      for (int i = 0; i < d2; ++i) cfma(acc[i], vr[i], T[((long)a * d2 + i) * B + b]);
+    }
+    double2* orow = out + ((l * B + b) * Q + q) * d2;
+    for (int o = 0; o < d2; ++o) {
+      double2 y = make_double2(0.0, 0.0);
+      for (int i = 0; i < d2; ++i) {
+        if (tout) cfma(y, acc[i], tout[i * d2 + o]);
+        else if (i == o) y = acc[i];
+      }
+      orow[o] = cscale(y, scale);
+    }
+  }
+}
+
+struct CapSet {
+  const double2* cap[kMaxEnvironments];
+  int bond[kMaxEnvironments];
+  int E;
+};
+
+// rho[o] = sum_{a_0..a_{E-1}} prod_e cap_e[a_e] v[a_0..a_{E-1}][o]  (one workgroup)
+__global__ void __launch_bounds__(256)
+dyn_readout_multi_kernel(const double2* v, CapSet cs, int d2, double2* rho) {
+  __shared__ double2 red[256];
+  long rows = 1;
+  for (int e = 0; e < cs.E; ++e) rows *= cs.bond[e];
+  for (int o = 0; o < d2; ++o) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (long f = threadIdx.x; f < rows; f += 256) {
+      long rem = f;
+      double2 wgt = make_double2(1.0, 0.0);
+      for (int e = cs.E - 1; e >= 0; --e) {
+        const int x = rem % cs.bond[e];
+        rem /= cs.bond[e];
+        wgt = cmul(wgt, cs.cap[e][x]);
+      }
+      cfma(acc, wgt, v[f * d2 + o]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) rho[o] = red[0];
+    __syncthreads();
+  }
+}
+
+// Device-resident process tensor of one environment, as the dynamics reads it.
+struct DynEnv {
+  std::vector<const double2*> sites;   // N sites [a][i][b]
+  std::vector<int> bonds;              // N + 1
+  std::vector<const double2*> caps;    // N + 1
+  double scale;                        // factor applied to every site
+  const double* tin;                   // host d2 x d2 or null
+  const double* tout;
+};
+
+// compute_dynamics of one system through E environments
+// (oqupy/system_dynamics.py:119-171, 654-700): per step the readout with the
+// product of the caps, P1, each environment's site with its transforms, P2.
+static void run_dynamics_multi(hipStream_t st, int d2, int N, const std::vector<DynEnv>& env,
+                               const double* props, const double* rho0, double* states_out) {
+  const int E = (int)env.size();
+  require(d2 >= 1 && d2 <= 16, PTE_ERR_ARG, "Liouville dimension must be 1..16");
+  require(E >= 1 && E <= kMaxEnvironments, PTE_ERR_ARG, "1 to 8 process tensors");
+  for (long i = 0; i < 2L * N * d2 * d2 * 2; ++i)
+    require(std::isfinite(props[i]), PTE_ERR_NUMERIC, "non-finite propagator");
+  auto bond = [&](int e, int k) { return env[e].bonds[k]; };
+  // largest state: while environment e is applied at step k the legs before
+  // e are at k + 1 and the others at k
+  long maxrows = 1;
+  for (int k = 0; k < N; ++k)
+    for (int e = 0; e <= E; ++e) {
+      long r = 1;
+      for (int f = 0; f < E; ++f) r *= bond(f, f < e ? k + 1 : k);
+      maxrows = std::max(maxrows, r);
+    }
+  require(maxrows * d2 <= (1L << 31), PTE_ERR_CAPACITY,
+          "product of the bond dimensions too large for the dynamics state");
+  auto up = [&](const double* h, long cnt) {
+    double2* p = nullptr;
+    HIP_CHECK(hipMallocAsync((void**)&p, std::max<long>(cnt, 1) * sizeof(double2), st));
+    HIP_CHECK(hipMemcpyAsync(p, h, cnt * sizeof(double2), hipMemcpyHostToDevice, st));
+    return p;
+  };
+  double2* dprops = up(props, 2L * N * d2 * d2);
+  std::vector<double2*> tin(E, nullptr), tout(E, nullptr);
+  for (int e = 0; e < E; ++e) {
+    if (env[e].tin) tin[e] = up(env[e].tin, (long)d2 * d2);
+    if (env[e].tout) tout[e] = up(env[e].tout, (long)d2 * d2);
+  }
+  double2 *v = nullptr, *w = nullptr, *rho = nullptr;
+  HIP_CHECK(hipMallocAsync((void**)&v, (size_t)maxrows * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&w, (size_t)maxrows * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMallocAsync((void**)&rho, (size_t)(N + 1) * d2 * sizeof(double2), st));
+  HIP_CHECK(hipMemcpyAsync(v, rho0, d2 * sizeof(double2), hipMemcpyHostToDevice, st));
+  auto caps_at = [&](int k) {
+    CapSet cs;
+    cs.E = E;
+    for (int e = 0; e < kMaxEnvironments; ++e) { cs.cap[e] = nullptr; cs.bond[e] = 1; }
+    for (int e = 0; e < E; ++e) { cs.cap[e] = env[e].caps[k]; cs.bond[e] = bond(e, k); }
+    return cs;
+  };
+  for (int k = 0; k < N; ++k) {
+    dyn_readout_multi_kernel<<<1, 256, 0, st>>>(v, caps_at(k), d2, rho + (long)k * d2);
+    const double2* P1 = dprops + (long)k * 2 * d2 * d2;
+    const double2* P2 = P1 + d2 * d2;
+    long rows = 1;
+    for (int e = 0; e < E; ++e) rows *= bond(e, k);
+    dyn_rows_kernel<<<grid_for(rows * d2), 256, 0, st>>>(v, rows, d2, P1, 0, w);
+    std::swap(v, w);
+    for (int e = 0; e < E; ++e) {
+      long L = 1, Q = 1;
+      for (int f = 0; f < e; ++f) L *= bond(f, k + 1);
+      for (int f = e + 1; f < E; ++f) Q *= bond(f, k);
+      const int A = bond(e, k), B = bond(e, k + 1);
+      if (tin[e]) {
+        dyn_rows_kernel<<<grid_for(L * A * Q * d2), 256, 0, st>>>(v, L * A * Q, d2, tin[e], 1,
+                                                                  w);
+        std::swap(v, w);
+      }
+      dyn_env_kernel<<<grid_for(L * B * Q), 256, 0, st>>>(v, L, A, Q, d2, env[e].sites[k], B,
+                                                          tout[e], env[e].scale, w);
+      std::swap(v, w);
+    }
+    rows = 1;
+    for (int e = 0; e < E; ++e) rows *= bond(e, k + 1);
+    dyn_rows_kernel<<<grid_for(rows * d2), 256, 0, st>>>(v, rows, d2, P2, 0, w);
+    std::swap(v, w);
+  }
+  dyn_readout_multi_kernel<<<1, 256, 0, st>>>(v, caps_at(N), d2, rho + (long)N * d2);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(states_out, rho, (size_t)(N + 1) * d2 * sizeof(double2),
+                           hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  (void)hipFreeAsync(v, st); (void)hipFreeAsync(w, st); (void)hipFreeAsync(rho, st);
+  (void)hipFreeAsync(dprops, st);
+  for (int e = 0; e < E; ++e) {
+    if (tin[e]) (void)hipFreeAsync(tin[e], st);
+    if (tout[e]) (void)hipFreeAsync(tout[e], st);
+  }
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
 // Runs the dynamics of S systems over device-resident sites [a][i][b]
 // (scaled by `scale`) and caps; props: S x N x 2 x d2 x d2, rho0: S x d2,
 // states_out: S x (N+1) x d2 (interleaved complex).
@@ -2519,6 +2703,116 @@ int pte_dynamics_run_engine_batch(pte_pt_engine* e, int nsys, const double* prop
     for (int k = 0; k <= N; ++k) caps[k] = pe.caps[k].p;
     run_dynamics(pe.st(), d2, N, sites, b, caps, props, tin, tout, rho0,
                  (double)pe.cfg.dimension, states_out, nsys);
+  });
+}
+
+int pte_dynamics_run_engines(pte_pt_engine* const* engines, int count, const double* props,
+                             const double* const* transforms_in,
+                             const double* const* transforms_out, const double* rho0,
+                             double* states_out) {
+  return guarded([&] {
+    require(engines && props && rho0 && states_out && transforms_in && transforms_out,
+            PTE_ERR_ARG, "null argument");
+    require(count >= 1 && count <= kMaxEnvironments, PTE_ERR_ARG, "1 to 8 process tensors");
+    require(engines[0] != nullptr, PTE_ERR_ARG, "null engine");
+    const int N = engines[0]->impl.cfg.num_steps, d = engines[0]->impl.cfg.dimension;
+    std::vector<DynEnv> env(count);
+    for (int e = 0; e < count; ++e) {
+      require(engines[e] != nullptr, PTE_ERR_ARG, "null engine");
+      PtEngine& pe = engines[e]->impl;
+      require(pe.step >= pe.cfg.num_steps, PTE_ERR_STATE, "process tensor not complete");
+      require(!pe.caps.empty(), PTE_ERR_STATE, "caps not computed");
+      require(pe.cfg.num_steps == N && pe.cfg.dimension == d, PTE_ERR_ARG,
+              "process tensors of different lengths or dimensions");
+      DynEnv& en = env[e];
+      en.sites.resize(N);
+      en.bonds.resize(N + 1);
+      en.caps.resize(N + 1);
+      for (int k = 0; k < N; ++k) {
+        require(pe.mps.sites[k].d[1] == d * d, PTE_ERR_ARG, "internal: site leg is not d^2");
+        en.sites[k] = pe.mps.sites[k].p;
+        en.bonds[k] = pe.mps.sites[k].d[0];
+      }
+      en.bonds[N] = pe.mps.sites[N - 1].d[2];
+      for (int k = 0; k <= N; ++k) en.caps[k] = pe.caps[k].p;
+      en.scale = (double)d;
+      en.tin = transforms_in[e];
+      en.tout = transforms_out[e];
+      // the sites are read on the first engine's stream
+      if (e > 0) HIP_CHECK(hipStreamSynchronize(pe.st()));
+    }
+    run_dynamics_multi(engines[0]->impl.st(), d * d, N, env, props, rho0, states_out);
+  });
+}
+
+int pte_dynamics_run_multi(int d2, int num_steps, int count, const int* bonds,
+                           const double* tensors, const double* caps, const double* props,
+                           const double* const* transforms_in,
+                           const double* const* transforms_out, const double* rho0,
+                           double* states_out) {
+  return guarded([&] {
+    require(bonds && tensors && caps && props && rho0 && states_out && transforms_in &&
+                transforms_out, PTE_ERR_ARG, "null argument");
+    require(num_steps >= 1, PTE_ERR_ARG, "num_steps must be >= 1");
+    require(count >= 1 && count <= kMaxEnvironments, PTE_ERR_ARG, "1 to 8 process tensors");
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    std::vector<double2*> owned;
+    std::vector<DynEnv> env(count);
+    long off = 0, coff = 0;
+    try {
+      for (int e = 0; e < count; ++e) {
+        DynEnv& en = env[e];
+        en.bonds.assign(bonds + (long)e * (num_steps + 1), bonds + (long)(e + 1) * (num_steps + 1));
+        require(en.bonds[0] == 1, PTE_ERR_ARG, "the first bond must be 1");
+        for (int k = 0; k <= num_steps; ++k)
+          require(en.bonds[k] >= 1, PTE_ERR_ARG, "bond dims must be positive");
+        en.sites.resize(num_steps);
+        en.caps.resize(num_steps + 1);
+        for (int k = 0; k < num_steps; ++k) {
+          // tensors come as [a][b][i]; the kernels read [a][i][b]
+          const int A = en.bonds[k], B = en.bonds[k + 1];
+          const long cnt = (long)A * B * d2;
+          std::vector<cplx> h(cnt);
+          for (int x = 0; x < A; ++x)
+            for (int y = 0; y < B; ++y)
+              for (int i = 0; i < d2; ++i) {
+                const long src = off + ((long)x * B + y) * d2 + i;
+                h[((long)x * d2 + i) * B + y] = cplx(tensors[2 * src], tensors[2 * src + 1]);
+              }
+          double2* p = nullptr;
+          HIP_CHECK(hipMallocAsync((void**)&p, cnt * sizeof(double2), st));
+          owned.push_back(p);
+          HIP_CHECK(hipMemcpyAsync(p, h.data(), cnt * sizeof(double2), hipMemcpyHostToDevice,
+                                   st));
+          HIP_CHECK(hipStreamSynchronize(st));
+          en.sites[k] = p;
+          off += cnt;
+        }
+        for (int k = 0; k <= num_steps; ++k) {
+          double2* p = nullptr;
+          HIP_CHECK(hipMallocAsync((void**)&p, en.bonds[k] * sizeof(double2), st));
+          owned.push_back(p);
+          HIP_CHECK(hipMemcpyAsync(p, caps + 2 * coff, en.bonds[k] * sizeof(double2),
+                                   hipMemcpyHostToDevice, st));
+          en.caps[k] = p;
+          coff += en.bonds[k];
+        }
+        en.scale = 1.0;
+        en.tin = transforms_in[e];
+        en.tout = transforms_out[e];
+      }
+      run_dynamics_multi(st, d2, num_steps, env, props, rho0, states_out);
+    } catch (...) {
+      (void)hipStreamSynchronize(st);
+      for (auto p : owned) (void)hipFreeAsync(p, st);
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    for (auto p : owned) (void)hipFreeAsync(p, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipStreamDestroy(st));
   });
 }
 

@@ -1,7 +1,8 @@
 """``compute_dynamics``: system dynamics through a process tensor.
 
 Restates ``compute_dynamics`` and its input parsing from
-``oqupy/system_dynamics.py:41-182, 478-559`` for one (or no) process tensor.
+``oqupy/system_dynamics.py:41-182, 478-559`` for no, one or several (up to 8)
+process tensors, each carrying its own bond leg of the state.
 Control operations are folded into the half-step propagators on the host
 (the post-measurement control of step k into the first half-step of step k,
 the pre-measurement control of step k+1 into the second half-step of step k,
@@ -9,9 +10,11 @@ that of step 0 into the initial state), which reproduces the reference's
 order of application. The per-step contraction -- first half-step propagator,
 PT-MPO tensor (with the basis transforms of a non-diagonal coupling), second
 half-step, and the cap readout of every state -- runs in the MI355X engine
-(``pte_dynamics_run`` / ``pte_dynamics_run_engine``).
+(``pte_dynamics_run`` / ``pte_dynamics_run_engine`` for one process tensor,
+``pte_dynamics_run_engines`` / ``pte_dynamics_run_multi`` for several).
 """
 
+import ctypes
 from typing import List, Optional, Text, Union
 
 import numpy as np
@@ -122,9 +125,9 @@ def compute_dynamics(
          control, record_all)
     process_tensors = [pt for pt in process_tensors
                        if not isinstance(pt, TrivialProcessTensor)]
-    if len(process_tensors) > 1:
-        raise NotImplementedError("More than one process tensor is not "
-                                  "supported by this package.")
+    if len(process_tensors) > 8:
+        raise NotImplementedError("At most 8 process tensors are supported "
+                                  "by this package.")
     d2 = hs_dim ** 2
     propagators = system.get_propagators(dt, start_time, subdiv_limit,
                                          liouvillian_epsrel)
@@ -140,36 +143,42 @@ def compute_dynamics(
     eng.require_gpu()
     lib = eng.load_library()
 
-    if process_tensors:
-        pt = process_tensors[0]
-        tin = None if pt.transform_in is None else eng.as_c128(pt.transform_in)
-        tout = None if pt.transform_out is None \
-            else eng.as_c128(pt.transform_out)
-        backend = getattr(pt, "engine", None)
-        if backend is not None and num_steps == len(pt):
-            eng.check(lib.pte_dynamics_run_engine(
-                backend.handle, eng.dptr(props), eng.dptr(tin),
-                eng.dptr(tout), eng.dptr(rho0), eng.dptr(states)))
-        else:
-            tensors, bonds, caps = [], [], []
-            for step in range(num_steps):
-                ten = pt.get_raw_mpo_tensor(step)
-                if ten.ndim != 3:
-                    raise NotImplementedError(
-                        "Only rank-3 PT-MPO tensors are supported.")
-                tensors.append(eng.as_c128(ten).reshape(-1))
-                bonds.append(ten.shape[0])
-            bonds.append(pt.get_raw_mpo_tensor(num_steps - 1).shape[1]
-                         if num_steps == len(pt) else
-                         pt.get_raw_mpo_tensor(num_steps).shape[0])
-            for step in range(num_steps + 1):
-                cap = pt.get_cap_tensor(step)
-                if cap is None:
-                    raise ValueError(f"Process tensor 0 has no cap tensor for "
-                                     f"step {step}.")
-                caps.append(eng.as_c128(cap).reshape(-1))
-            _run_host(lib, d2, num_steps, bonds, tensors, caps, props, tin,
-                      tout, rho0, states)
+    tins = [None if pt.transform_in is None else eng.as_c128(pt.transform_in)
+            for pt in process_tensors]
+    touts = [None if pt.transform_out is None
+             else eng.as_c128(pt.transform_out) for pt in process_tensors]
+    resident = all(getattr(pt, "engine", None) is not None
+                   and num_steps == len(pt) for pt in process_tensors)
+    if len(process_tensors) == 1 and resident:
+        eng.check(lib.pte_dynamics_run_engine(
+            process_tensors[0].engine.handle, eng.dptr(props),
+            eng.dptr(tins[0]), eng.dptr(touts[0]), eng.dptr(rho0),
+            eng.dptr(states)))
+    elif len(process_tensors) == 1:
+        tensors, bonds, caps = _host_arrays(process_tensors[0], 0, num_steps)
+        _run_host(lib, d2, num_steps, bonds, tensors, caps, props, tins[0],
+                  touts[0], rho0, states)
+    elif process_tensors and resident:
+        handles = (ctypes.c_void_p * len(process_tensors))(
+            *[pt.engine.handle.value for pt in process_tensors])
+        eng.check(lib.pte_dynamics_run_engines(
+            handles, len(process_tensors), eng.dptr(props),
+            _pointer_array(tins), _pointer_array(touts), eng.dptr(rho0),
+            eng.dptr(states)))
+    elif process_tensors:
+        tensors, bonds, caps = [], [], []
+        for index, pt in enumerate(process_tensors):
+            ten, bnd, cap = _host_arrays(pt, index, num_steps)
+            tensors += ten
+            bonds += bnd
+            caps += cap
+        bonds_arr = np.ascontiguousarray(bonds, dtype=np.int32)
+        eng.check(lib.pte_dynamics_run_multi(
+            d2, num_steps, len(process_tensors), eng.iptr(bonds_arr),
+            eng.dptr(np.ascontiguousarray(np.concatenate(tensors))),
+            eng.dptr(np.ascontiguousarray(np.concatenate(caps))),
+            eng.dptr(props), _pointer_array(tins), _pointer_array(touts),
+            eng.dptr(rho0), eng.dptr(states)))
     else:
         ones = [np.ones(d2, dtype=np.complex128) for _ in range(num_steps)]
         caps = [np.ones(1, dtype=np.complex128)] * (num_steps + 1)
@@ -183,6 +192,37 @@ def compute_dynamics(
     # the reference records one state and dates it start_time + 1 * dt
     # (oqupy/system_dynamics.py:173-180)
     return Dynamics(times=[start_time + 1 * dt], states=[state_list[-1]])
+
+
+def _host_arrays(pt: BaseProcessTensor, index: int, num_steps: int):
+    """Host copies of the first ``num_steps`` PT-MPO tensors ([a][b][i],
+    flattened), the bond dimensions and the caps of one process tensor."""
+    tensors, bonds, caps = [], [], []
+    for step in range(num_steps):
+        ten = pt.get_raw_mpo_tensor(step)
+        if ten.ndim != 3:
+            raise NotImplementedError(
+                "Only rank-3 PT-MPO tensors are supported.")
+        tensors.append(eng.as_c128(ten).reshape(-1))
+        bonds.append(ten.shape[0])
+    bonds.append(pt.get_raw_mpo_tensor(num_steps - 1).shape[1]
+                 if num_steps == len(pt) else
+                 pt.get_raw_mpo_tensor(num_steps).shape[0])
+    for step in range(num_steps + 1):
+        cap = pt.get_cap_tensor(step)
+        if cap is None:
+            raise ValueError(f"Process tensor {index} has no cap tensor for "
+                             f"step {step}.")
+        caps.append(eng.as_c128(cap).reshape(-1))
+    return tensors, bonds, caps
+
+
+def _pointer_array(arrays):
+    """``const double**`` over complex128 arrays (NULL entries for None)."""
+    ptrs = (eng._DP * len(arrays))()
+    for i, arr in enumerate(arrays):
+        ptrs[i] = eng.dptr(arr) if arr is not None else eng._DP()
+    return ptrs
 
 
 def _run_host(lib, d2, num_steps, bonds, tensors, caps, props, tin, tout,

@@ -168,6 +168,114 @@ def test_compute_dynamics_host_tensors():
     assert np.abs(dyn.states - np.array(pre)).max() < STATE_TOL
 
 
+def test_compute_dynamics_multi_env_golden():
+    """The reference's multi-environment case through the engine: two
+    device-resident process tensors at alpha against one at 2 alpha
+    (tests/physics/multi_environments_test.py:24-68, decimal 5), and both
+    against the oracle's contraction of the same environments."""
+    _announce("dynamics multi-env golden")
+    from oracle.dynamics import compute_dynamics
+    from oracle.pt_tempo import PtTempoOracle, compute_caps
+    g = models.goldens()["multi_env"]
+    system = oqupy.System(op.sigma("x"))
+    rho0 = op.spin_dm("z+")
+    pts, refs = {}, {}
+    for key in ("alpha", "alpha2"):
+        corr = oqupy.PowerLawSD(alpha=g[key], zeta=1.0, cutoff=g["cutoff"],
+                                cutoff_type="exponential",
+                                temperature=g["temperature"])
+        bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
+        params = oqupy.TempoParameters(dt=g["dt"], dkmax=g["dkmax"],
+                                       epsrel=g["epsrel"])
+        pts[key] = oqupy.pt_tempo_compute(bath, 0.0, g["t_end"], params)
+        infl = models.CachedInfluence(bath, params)
+        ref = PtTempoOracle(2, infl, np.ones(4), g["num_steps"], g["dkmax"],
+                            g["epsrel"])
+        ref.compute()
+        tens = ref.mpo_tensors()
+        refs[key] = (tens, compute_caps(tens, 2))
+    two = oqupy.compute_dynamics(system=system, initial_state=rho0,
+                                 process_tensor=[pts["alpha"], pts["alpha"]])
+    one = oqupy.compute_dynamics(system=system, initial_state=rho0,
+                                 process_tensor=[pts["alpha2"]])
+    np.testing.assert_almost_equal(np.array(two.states),
+                                   np.array(one.states), decimal=g["decimal"])
+    props = system.get_propagators(g["dt"], 0.0, None, None)
+    ref_two = compute_dynamics(props, rho0, g["num_steps"],
+                               [refs["alpha"][0]] * 2, [refs["alpha"][1]] * 2)
+    assert np.abs(np.array(two.states) - np.array(ref_two)).max() < STATE_TOL
+
+
+def _three_environments(n):
+    """Three baths on a spin: sigma_z couplings at two temperatures and a
+    sigma_x coupling (basis transforms); engine PTs and oracle tensors."""
+    from oracle.pt_tempo import PtTempoOracle, compute_caps
+    specs = [("z", 0.2, 5.0, 0.2, 1e-7), ("z", 0.1, 3.0, 0.0, 1e-7),
+             ("x", 0.15, 4.0, 0.5, 1e-6)]
+    pts, tensors, caps = [], [], []
+    for axis, alpha, cutoff, temp, epsrel in specs:
+        corr = oqupy.PowerLawSD(alpha, 1.0, cutoff, temperature=temp)
+        bath = oqupy.Bath(0.5 * op.sigma(axis), corr)
+        params = oqupy.TempoParameters(dt=0.05, dkmax=4, epsrel=epsrel)
+        pts.append(oqupy.pt_tempo_compute(bath, 0.0, n * 0.05 + 1e-11,
+                                          params))
+        infl = models.CachedInfluence(bath, params)
+        ref = PtTempoOracle(2, infl, np.ones(4), n, 4, epsrel)
+        ref.compute()
+        tens = ref.mpo_tensors()
+        tensors.append(tens)
+        caps.append(compute_caps(tens, 2))
+    return pts, tensors, caps
+
+
+def test_compute_dynamics_three_environments():
+    """Three device-resident process tensors (one with basis transforms),
+    with controls, against the oracle; the same environments as host
+    tensors (SimpleProcessTensor) through the host-array entry point."""
+    _announce("dynamics three environments")
+    from oracle.dynamics import compute_dynamics
+    n = 12
+    pts, tensors, caps = _three_environments(n)
+    assert [len(pt) for pt in pts] == [n] * 3
+    system = oqupy.System(0.5 * op.sigma("x") + 0.3 * op.sigma("z"))
+    rho0 = op.spin_dm("y+")
+    control = oqupy.Control(2)
+    x = op.sigma("x")
+    control.add_single(5, op.left_right_super(x, x.conj().T))
+    dyn = oqupy.compute_dynamics(system=system, initial_state=rho0,
+                                 process_tensor=pts, control=control)
+    props = system.get_propagators(0.05, 0.0, None, None)
+    transforms = [(pt.transform_in, pt.transform_out) for pt in pts]
+    ref = compute_dynamics(props, rho0, n, tensors, caps,
+                           transforms=transforms,
+                           controls=lambda s: control.get_controls(s, dt=0.05))
+    assert np.abs(dyn.states - np.array(ref)).max() < STATE_TOL
+    # host tensors: the oracle's, in SimpleProcessTensors with the same
+    # transforms, over a prefix of 9 steps
+    hosts = []
+    for k in range(3):
+        pt = oqupy.SimpleProcessTensor(2, dt=0.05,
+                                       transform_in=pts[k].transform_in,
+                                       transform_out=pts[k].transform_out)
+        for step, t in enumerate(tensors[k]):
+            pt.set_mpo_tensor(step, t)
+        for step, c in enumerate(caps[k]):
+            pt.set_cap_tensor(step, c)
+        hosts.append(pt)
+    dyn = oqupy.compute_dynamics(system=system, initial_state=rho0,
+                                 process_tensor=hosts, num_steps=9)
+    ref = compute_dynamics(props, rho0, 9, tensors, caps,
+                           transforms=transforms)
+    assert np.abs(dyn.states - np.array(ref)).max() < STATE_TOL
+    # a mix of a device-resident and a host process tensor
+    dyn = oqupy.compute_dynamics(system=system, initial_state=rho0,
+                                 process_tensor=[pts[0], hosts[2]])
+    ref = compute_dynamics(props, rho0, n, [tensors[0], tensors[2]],
+                           [caps[0], caps[2]],
+                           transforms=[transforms[0], transforms[2]])
+    assert np.abs(dyn.states - np.array(ref)).max() < STATE_TOL
+
+
 def _oracle_tempo_spread(system, bath, params, rho0, n):
     """Oracle TEMPO states and the spread between two backward-stable LAPACK
     drivers (gesdd vs gesvd). TEMPO's truncation amplifies rounding-level
