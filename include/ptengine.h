@@ -258,6 +258,44 @@ int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap,
 int pte_contract(int precision, int m, int n, int k, const double* A, const double* B,
                  double* C);
 
+/* ---- learned truncation-rank form (no counterpart in the reference: the
+ * reference always runs the full SVD of split_node_full_svd,
+ * oqupy/backends/node_array.py:262, 285, 541) ----
+ * The pivoted QR that starts every truncated SVD of the zip-up and the sweep
+ * is capped. mode 0: no cap (exact; the default). mode 1: the bond's
+ * previous step count + oversample. mode 2: the step count the MLP `params`
+ * (pte_rank_param_count() floats) predicts, times (1 + margin), +
+ * oversample. mode 3: stop at the acceptance threshold itself. A capped
+ * factorisation is accepted when its residual is at most
+ * theta * epsrel * |beta_0| (the largest column norm); the residual then
+ * counts as discarded weight in the cut. Otherwise the pivoted QR resumes and
+ * runs to the exact stop. alpha and temperature are features of the bath. */
+int pte_pt_set_rank_mode(pte_pt_engine* e, int mode, const float* params, double alpha,
+                         double temperature, int oversample, double margin, double theta);
+/* Record up to `capacity` training records (features, target log2(need) / 8,
+ * need, kept rank, min(m, n): PTE_RANK_RECORD floats each) of the truncations
+ * that follow; need = the pivoted-QR steps that reached the threshold. */
+int pte_pt_rank_trace(pte_pt_engine* e, int capacity);
+/* Copy up to max_records records into out (NULL: count only). */
+int pte_pt_get_rank_trace(pte_pt_engine* e, float* out, int max_records, int* count);
+/* Truncations seen, capped, fallen back to the exact SVD; pivoted-QR steps. */
+int pte_pt_rank_stats(pte_pt_engine* e, long* svds, long* capped, long* fallbacks,
+                      long* qr_steps);
+
+#define PTE_RANK_RECORD 28
+/* The rank predictor's network (24 features -> 3 x 128 tanh -> 1, FP32). */
+int pte_rank_param_count(void);
+int pte_rank_feature_count(void);
+/* yhat[i] = network(X[i]) for n feature rows. */
+int pte_rank_mlp_predict(const float* params, const float* X, int n, float* yhat);
+/* loss = mean (network(X[i]) - y[i])^2 and its gradient w.r.t. params. */
+int pte_rank_mlp_grad(const float* params, const float* X, const float* y, int n, float* grad,
+                      float* loss);
+/* Adam on mini-batches of a seeded shuffle; params updated in place,
+ * epoch_loss (optional) receives the mean loss of every epoch. */
+int pte_rank_mlp_train(float* params, const float* X, const float* y, int n, int epochs,
+                       int batch, float lr, unsigned seed, float* epoch_loss);
+
 const char* pte_last_error(void);
 
 /* Library version string. */

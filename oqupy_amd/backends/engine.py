@@ -56,6 +56,7 @@ def library_path() -> str:
 _P = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
 
 _SIGNATURES = {
     "pte_last_error": (ctypes.c_char_p, []),
@@ -111,6 +112,19 @@ _SIGNATURES = {
     "pte_dynamics_run_engines": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, _DP,
                                                 ctypes.POINTER(_DP),
                                                 ctypes.POINTER(_DP), _DP, _DP]),
+    "pte_pt_set_rank_mode": (ctypes.c_int, [_P, ctypes.c_int, _FP, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double]),
+    "pte_pt_rank_trace": (ctypes.c_int, [_P, ctypes.c_int]),
+    "pte_pt_get_rank_trace": (ctypes.c_int, [_P, _FP, ctypes.c_int, _IP]),
+    "pte_pt_rank_stats": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_long)] * 4),
+    "pte_rank_param_count": (ctypes.c_int, []),
+    "pte_rank_feature_count": (ctypes.c_int, []),
+    "pte_rank_mlp_predict": (ctypes.c_int, [_FP, _FP, ctypes.c_int, _FP]),
+    "pte_rank_mlp_grad": (ctypes.c_int, [_FP, _FP, _FP, ctypes.c_int, _FP, _FP]),
+    "pte_rank_mlp_train": (ctypes.c_int, [_FP, _FP, _FP, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_float, ctypes.c_uint,
+                                          _FP]),
     "pte_dynamics_run_multi": (ctypes.c_int, [ctypes.c_int] * 3 + [_IP, _DP, _DP, _DP,
                                               ctypes.POINTER(_DP),
                                               ctypes.POINTER(_DP), _DP, _DP]),
@@ -186,6 +200,13 @@ def iptr(array: np.ndarray):
     return array.ctypes.data_as(_IP)
 
 
+def fptr(array: Optional[np.ndarray]):
+    """``float*`` view of a float32 array, or NULL."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(_FP)
+
+
 def parse_backend_config(config: Optional[dict]) -> dict:
     """Validate a backend configuration dictionary; only the MI355X engine
     exists in this package."""
@@ -202,7 +223,29 @@ def parse_backend_config(config: Optional[dict]) -> dict:
     chi_cap = int(cfg.get("chi_cap", DEFAULT_CHI_CAP))
     if chi_cap < 1:
         raise ValueError("chi_cap must be positive.")
-    return {"backend": backend, "precision": precision, "chi_cap": chi_cap}
+    out = {"backend": backend, "precision": precision, "chi_cap": chi_cap}
+    # learned truncation-rank form (PT-TEMPO only)
+    predictor = cfg.get("rank_predictor", None)
+    if predictor is not None and predictor not in ("previous", "adaptive") \
+            and getattr(predictor, "params", None) is None:
+        raise ValueError("rank_predictor must be None, 'previous', "
+                         "'adaptive' or a RankPredictor.")
+    if predictor is not None and precision != "c128":
+        raise ValueError("The learned rank form runs with precision 'c128'.")
+    trace = int(cfg.get("rank_trace", 0))
+    if trace < 0:
+        raise ValueError("rank_trace must be >= 0.")
+    oversample = int(cfg.get("rank_oversample", 8))
+    margin = float(cfg.get("rank_margin", 0.1))
+    theta = float(cfg.get("rank_theta", 0.02))
+    if oversample < 0 or margin < 0.0 or not 0.0 < theta <= 1.0:
+        raise ValueError("rank_oversample >= 0, rank_margin >= 0 and "
+                         "0 < rank_theta <= 1 are required.")
+    bath = tuple(float(v) for v in cfg.get("rank_bath", (0.0, 0.0)))
+    out.update(rank_predictor=predictor, rank_trace=trace,
+               rank_oversample=oversample, rank_margin=margin,
+               rank_theta=theta, rank_bath=bath)
+    return out
 
 
 def contract(a: np.ndarray, b: np.ndarray, precision: str = "c128"):

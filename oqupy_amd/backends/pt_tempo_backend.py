@@ -86,8 +86,58 @@ class PtTempoBackend:
                                         dtype=np.int32)
             eng.check(self._lib.pte_pt_set_degeneracy_maps(
                 self._handle, eng.iptr(nmap), eng.iptr(wmap), int(nmap.size)))
+        self._configure_rank()
         eng.check(self._lib.pte_pt_initialize(self._handle))
         self._step = 1
+
+    def _configure_rank(self) -> None:
+        """Learned truncation-rank form: trace recording and the cap of the
+        pivoted QRs (the MLP's prediction, the bond's previous step count, or
+        the adaptive stop at the acceptance threshold)."""
+        cfg = self._config
+        if cfg["rank_trace"] > 0:
+            eng.check(self._lib.pte_pt_rank_trace(self._handle,
+                                                  cfg["rank_trace"]))
+        predictor = cfg["rank_predictor"]
+        if predictor is None:
+            return
+        alpha, temperature = cfg["rank_bath"]
+        params = None
+        if predictor == "previous":
+            mode = 1
+        elif predictor == "adaptive":
+            mode = 3
+        else:
+            params = np.ascontiguousarray(predictor.params, dtype=np.float32)
+            if params.size != self._lib.pte_rank_param_count():
+                raise ValueError("RankPredictor parameters of the wrong size.")
+            mode = 2
+        eng.check(self._lib.pte_pt_set_rank_mode(
+            self._handle, mode, eng.fptr(params), alpha, temperature,
+            cfg["rank_oversample"], cfg["rank_margin"], cfg["rank_theta"]))
+
+    def rank_trace(self) -> ndarray:
+        """Training records of the truncations so far: one row per SVD with
+        the 24 features, the target log2(need)/8, need (the pivoted-QR steps
+        that reached the acceptance threshold), the kept rank and min(m, n)."""
+        count = ctypes.c_int()
+        eng.check(self._lib.pte_pt_get_rank_trace(self._handle, None, 0,
+                                                  ctypes.byref(count)))
+        width = 28
+        out = np.zeros((count.value, width), dtype=np.float32)
+        if count.value:
+            eng.check(self._lib.pte_pt_get_rank_trace(
+                self._handle, eng.fptr(out), count.value, ctypes.byref(count)))
+        return out
+
+    def rank_stats(self) -> Dict[str, int]:
+        """Truncations seen by the learned form, how many were capped, how
+        many fell back to the exact SVD, and the pivoted-QR steps taken."""
+        vals = [ctypes.c_long() for _ in range(4)]
+        eng.check(self._lib.pte_pt_rank_stats(
+            self._handle, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(("svds", "capped", "fallbacks", "qr_steps"),
+                        (v.value for v in vals)))
 
     def compute_step(self) -> bool:
         """One PT-TEMPO step (pt_tempo_backend.py:192-282); returns whether

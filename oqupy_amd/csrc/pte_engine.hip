@@ -27,6 +27,7 @@
 #include "ptengine.h"
 #include "pte_device.h"
 #include "pte_kernels.h"
+#include "pte_rank.h"
 
 namespace pte {
 
@@ -176,6 +177,69 @@ static inline int k_bound(int m, int n) { return std::max(1, std::min(m, n)); }
 
 struct Batcher;
 
+// Learned truncation-rank form: per-bond history on the device, the
+// predictor's parameters and the statistics of the capped SVDs.
+struct RankState {
+  int mode = 0;                 // 0 exact, 1 previous steps + oversample, 2 MLP,
+                                // 3 adaptive (stop at the acceptance threshold)
+  float* params = nullptr;      // device, kRankParams (mode 2)
+  float alpha = 0.f, temperature = 0.f;
+  int dkmax = 1, num_steps = 1;
+  int oversample = 8;
+  float margin = 0.1f;
+  double theta = 0.02;          // accept when residual <= theta * epsrel * |beta_0|
+  float step_frac = 0.f;
+  float* hist = nullptr;
+  int* hist_rank = nullptr;
+  int* cap_d = nullptr;
+  double* out_d = nullptr;      // stop summary of the last pivoted QR (4 doubles)
+  double* out_h = nullptr;
+  float* rec = nullptr;
+  int* rec_count = nullptr;
+  int rec_cap = 0;
+  long svds = 0, capped = 0, fallbacks = 0, qr_steps = 0;
+
+  void init(hipStream_t st) {
+    HIP_CHECK(hipMalloc((void**)&hist, sizeof(float) * kRankKeys * kRankHist));
+    HIP_CHECK(hipMalloc((void**)&hist_rank, sizeof(int) * kRankKeys));
+    HIP_CHECK(hipMalloc((void**)&cap_d, sizeof(int)));
+    HIP_CHECK(hipMalloc((void**)&out_d, 4 * sizeof(double)));
+    HIP_CHECK(hipHostMalloc((void**)&out_h, 4 * sizeof(double), hipHostMallocDefault));
+    HIP_CHECK(hipMalloc((void**)&rec_count, sizeof(int)));
+    HIP_CHECK(hipMemsetAsync(hist_rank, 0, sizeof(int) * kRankKeys, st));
+    HIP_CHECK(hipMemsetAsync(rec_count, 0, sizeof(int), st));
+    HIP_CHECK(hipMemsetAsync(out_d, 0, 4 * sizeof(double), st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  }
+  void release() {
+    (void)hipFree(params); (void)hipFree(hist); (void)hipFree(hist_rank);
+    (void)hipFree(cap_d); (void)hipFree(out_d); (void)hipHostFree(out_h);
+    (void)hipFree(rec); (void)hipFree(rec_count);
+    params = nullptr; hist = nullptr; hist_rank = nullptr; cap_d = nullptr;
+    out_d = nullptr; out_h = nullptr; rec = nullptr; rec_count = nullptr;
+  }
+  RankQuery query(int key, int m, int n, float epsrel) const {
+    RankQuery q;
+    q.key = key;
+    q.step_frac = step_frac;
+    q.kind = (float)(key / kRankKeysPerKind);
+    q.rel_frac = (float)(key % kRankKeysPerKind) / (float)std::max(dkmax, 1);
+    q.log_n = std::log2((float)std::max(1, std::min(m, n))) / 8.0f;
+    q.alpha = alpha;
+    q.temperature = temperature;
+    q.log_eps = std::log10(epsrel) / 10.0f;
+    q.m = m;
+    q.n = n;
+    return q;
+  }
+};
+
+// history key of a bond: kind 0 zip-up, 1 sweep; rel = distance of the
+// site from the newest one
+static int rank_key(int kind, int rel) {
+  return kind * kRankKeysPerKind + std::max(0, std::min(rel, kRankKeysPerKind - 1));
+}
+
 struct Workspace {
   hipStream_t stream = nullptr;
   // stream of the SVD phases: `stream`, or the batch stream while a batched
@@ -226,6 +290,9 @@ struct Workspace {
   // optional per-SVD log (environment variable PTE_SVD_TRACE = file path):
   // m, n, pivoted-QR rank (0 without QR), kept rank, sweeps, QR ms, Jacobi ms
   FILE* trace = nullptr;
+  // learned truncation-rank form (null when off)
+  RankState* rank = nullptr;
+  const double* trunc_extra2 = nullptr;   // residual folded into the cut (capped QR)
   int tr_m = 0, tr_n = 0, tr_r = 0;
   float tr_qr_ms = 0.f;
   double jac_ms = 0.0, jac_flops = 0.0, jac_bytes = 0.0;
@@ -380,7 +447,8 @@ struct Workspace {
   void jacobi_finish(int m, int n) {
     column_norm_kernel<<<n, 256, 0, svd_stream>>>(X.p, m, n, m, sigma);
     rank_sort_kernel<<<(n + 255) / 256, 256, 0, svd_stream>>>(sigma, n, perm, sorted);
-    truncate_kernel<<<1, 1024, 0, svd_stream>>>(sorted, n, epsrel, info_d, chi_cap);
+    truncate_kernel<<<1, 1024, 0, svd_stream>>>(sorted, n, epsrel, info_d, chi_cap,
+                                                 trunc_extra2);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
                              svd_stream));
@@ -463,11 +531,97 @@ struct Workspace {
   //   left singular vector j = Q W[:, perm[j]]            (emit_left_q)
   //   row j of S V^H at column piv[s] = conj(Z[s, perm[j]])  (emit_svh)
   // Returns the kept rank k.
-  int svd_qrcp(int m, int n) {
+  // `key` >= 0 names the bond (rank_key) for the learned form: its history
+  // is updated and, with a predictor, the pivoted QR is capped at the
+  // predicted step count + oversample. The capped factorisation is accepted
+  // when its residual is at most theta * epsrel * |beta_0| (|beta_0|, the
+  // largest column norm, is a lower bound of s0), and the residual then
+  // counts as discarded weight in the cut; otherwise the pivoted QR resumes
+  // where it stopped and runs to the exact stop. The step count the history
+  // and the trace keep is the first one whose residual met that threshold
+  // (mode 3 stops there without a prediction).
+  int svd_qrcp(int m, int n, int key = -1) {
     if (batcher) return batched_svd(true, m, n);
     if (small_ok(m, n)) return svd_small(m, n);
+    const bool learn = rank != nullptr && key >= 0;
+    const bool capped = learn && rank->mode > 0;
+    const RankQuery q = learn ? rank->query(key, m, n, (float)epsrel) : RankQuery();
+    if (capped && rank->mode != 3) {
+      RankPredictArgs pa;
+      pa.params = rank->mode == 2 ? rank->params : nullptr;
+      pa.hist = rank->hist;
+      pa.hist_rank = rank->hist_rank;
+      pa.q = q;
+      pa.oversample = rank->oversample;
+      pa.margin = rank->margin;
+      pa.nmax = std::min(m, n);
+      pa.cap = rank->cap_d;
+      rank_predict_kernel<<<1, kRankWidth, 0, svd_stream>>>(pa);
+    }
+    float ms = 0.f;
+    const bool predicted = capped && rank->mode != 3;
+    int r = qrcp_run(m, n, predicted ? rank->cap_d : nullptr, 0, 0.0, &ms,
+                     capped && rank->mode == 3);
+    bool folded = false;
+    if (capped) {
+      rank->capped += 1;
+      const double* o = rank->out_h;   // resid2, ||M||_F^2, |beta_0|, mark
+      const double thr = rank->theta * epsrel * o[2];
+      if (o[0] <= thr * thr) {
+        folded = o[0] > 0.0;
+      } else {
+        rank->fallbacks += 1;
+        const double eps = 2.220446049250313e-16;
+        float ms2 = 0.f;
+        r = qrcp_run(m, n, nullptr, r, eps * eps * o[1], &ms2);
+        ms += ms2;
+      }
+    }
+    if (rank) rank->qr_steps += r;
+    qr_finish_host(m, n, r, ms);
+    build_rh_kernel<<<grid_for((long)n * r), 256, 0, svd_stream>>>(Q.p, m, n, qr_piv,
+                                                                   qr_beta, qr_rank_d, X.p);
+    trunc_extra2 = folded ? rank->out_d : nullptr;
+    int k;
+    try {
+      k = svd(n, r);
+    } catch (...) {
+      trunc_extra2 = nullptr;
+      throw;
+    }
+    trunc_extra2 = nullptr;
+    if (learn) {
+      rank->svds += 1;
+      RankRecordArgs ra;
+      ra.hist = rank->hist;
+      ra.hist_rank = rank->hist_rank;
+      ra.q = q;
+      ra.sorted = sorted;
+      ra.info = info_d;
+      ra.need = rank->out_d + 3;
+      ra.rec = rank->rec;
+      ra.rec_count = rank->rec_count;
+      ra.rec_cap = rank->rec_cap;
+      rank_record_kernel<<<1, kRankWidth, 0, svd_stream>>>(ra);
+      HIP_CHECK(hipGetLastError());
+    }
+    return k;
+  }
+
+  // one launch of the pivoted QR of Q (m x n): capped at *max_steps when
+  // given, resumed after `start` steps when start > 0 (stop threshold
+  // drop_abs2 on the squared residual). Returns the steps taken in total;
+  // the stop summary goes to rank->out_h when the learned form is on.
+  int qrcp_run(int m, int n, const int* max_steps, int start, double drop_abs2, float* ms,
+               bool stop_at_mark = false) {
     const int nblk = qr_natural_grid(n);
     QrcpArgs qa = qr_prepare(m, n, nblk);
+    qa.max_steps = max_steps;
+    qa.start = start;
+    qa.drop_abs2 = drop_abs2;
+    qa.out = rank ? rank->out_d : nullptr;
+    qa.mark_rel = rank ? rank->theta * epsrel : 0.0;
+    qa.stop_at_mark = stop_at_mark ? 1 : 0;
     HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
     void* kargs[] = {&qa};
     HIP_CHECK(hipEventRecord(evq0, svd_stream));
@@ -477,11 +631,12 @@ struct Workspace {
     HIP_CHECK(hipEventRecord(evq1, svd_stream));
     HIP_CHECK(hipMemcpyAsync(qr_rank_h, qr_rank_d, sizeof(int), hipMemcpyDeviceToHost,
                              svd_stream));
+    if (rank)
+      HIP_CHECK(hipMemcpyAsync(rank->out_h, rank->out_d, 4 * sizeof(double),
+                               hipMemcpyDeviceToHost, svd_stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, evq0, evq1));
-    const int r = qr_finish(m, n, ms);
-    return svd(n, r);
+    HIP_CHECK(hipEventElapsedTime(ms, evq0, evq1));
+    return *qr_rank_h;
   }
 
   // ---- small SVDs: one workgroup, LDS resident (small_svd_kernel) ----
@@ -553,6 +708,12 @@ struct Workspace {
     qa.tau = qr_tau; qa.vscale = qr_vscale; qa.beta = qr_beta;
     qa.piv = qr_piv; qa.done = qr_done; qa.cand = qr_cand; qa.rank = qr_rank_d;
     qa.bar = bar;
+    qa.max_steps = nullptr;
+    qa.out = nullptr;
+    qa.start = 0;
+    qa.drop_abs2 = 0.0;
+    qa.mark_rel = 0.0;
+    qa.stop_at_mark = 0;
     return qa;
   }
 
@@ -1144,7 +1305,7 @@ struct Chain {
         // M = A as (Dl*P) x Dr, column-major copy for the pivoted QR
         ws->Q.ensure(cnt, st());
         transpose_kernel<<<grid_for(cnt), 256, 0, st()>>>(A.p, m, Dr, ws->Q.p);
-        k = ws->svd_qrcp(m, Dr);
+        k = ws->svd_qrcp(m, Dr, rank_key(1, (int)sites.size() - 1 - i));
         // new site [l][p][j]: row i = (l,p) -> i*k + j
         U = dalloc(Dl, P, k, 1, st());
         ws->emit_left_q(1, /*dj*/ 1, 0, /*da*/ 0, 1, 0, 0, U.p, k);
@@ -1275,7 +1436,7 @@ struct Chain {
         ma.out = ws->Q.p;
         ma.ldx = ncols;
         mpo_apply_kernel<<<grid_for(total), 256, 0, st()>>>(ma);
-        k = ws->svd_qrcp(ncols, mrows);
+        k = ws->svd_qrcp(ncols, mrows, rank_key(0, (int)sites.size() - 1 - ia));
         // new site [j][s][x]: row i = x*S + s (a = x, b = s)
         U = dalloc(k, S, Dx, 1, st());
         ws->emit_left_q(S, /*dj*/ (long)S * Dx, 0, /*da*/ 1, 0, /*db*/ Dx, 0, U.p, k);
@@ -1526,6 +1687,7 @@ struct PtEngine {
     require(step < cfg.num_steps, PTE_ERR_STATE, "process tensor already complete");
     step += 1;
     const bool end_phase = step > cfg.num_steps - num_infl + 1;
+    if (ws.rank) ws.rank->step_frac = (float)step / (float)cfg.num_steps;
     begin_stats();
     if (end_phase) {
       require(mpo.size() >= 2, PTE_ERR_STATE, "internal: MPO too short to shrink");
@@ -1625,7 +1787,23 @@ struct PtEngine {
     mps.clear();
     for (auto& nd : mpo) free_node(nd, st());
     mpo.clear();
+    if (ws.rank) {
+      (void)hipStreamSynchronize(st());
+      ws.rank->release();
+      delete ws.rank;
+      ws.rank = nullptr;
+    }
     ws.destroy();
+  }
+
+  RankState& rank_state() {
+    if (!ws.rank) {
+      ws.rank = new RankState();
+      ws.rank->init(st());
+      ws.rank->dkmax = std::max(cfg.dkmax, 1);
+      ws.rank->num_steps = cfg.num_steps;
+    }
+    return *ws.rank;
   }
 };
 
@@ -2202,6 +2380,32 @@ static int guarded(F&& f) {
   }
 }
 
+// device buffers of one call of the rank predictor's network, freed on exit
+struct MlpBuffers {
+  hipStream_t st = nullptr;
+  std::vector<void*> owned;
+  template <class T>
+  T* alloc(size_t n) {
+    T* p = nullptr;
+    HIP_CHECK(hipMallocAsync((void**)&p, std::max<size_t>(n, 1) * sizeof(T), st));
+    owned.push_back(p);
+    return p;
+  }
+  template <class T>
+  T* upload(const T* h, size_t n) {
+    T* p = alloc<T>(n);
+    HIP_CHECK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    return p;
+  }
+  MlpBuffers() { HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
+  ~MlpBuffers() {
+    (void)hipStreamSynchronize(st);
+    for (void* p : owned) (void)hipFreeAsync(p, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
+};
+
 }  // namespace pte
 
 using namespace pte;
@@ -2359,6 +2563,202 @@ int pte_pt_svd_kernel_stats(pte_pt_engine* e, long* count, double* ms, double* f
     if (ms) *ms = e->impl.ws.jac_ms;
     if (flops) *flops = e->impl.ws.jac_flops;
     if (bytes) *bytes = e->impl.ws.jac_bytes;
+  });
+}
+
+int pte_pt_set_rank_mode(pte_pt_engine* e, int mode, const float* params, double alpha,
+                         double temperature, int oversample, double margin, double theta) {
+  return guarded([&] {
+    require(e != nullptr, PTE_ERR_ARG, "null engine");
+    require(mode >= 0 && mode <= 3, PTE_ERR_ARG, "rank mode must be 0..3");
+    require(mode != 2 || params != nullptr, PTE_ERR_ARG, "the MLP mode needs parameters");
+    require(oversample >= 0 && margin >= 0.0 && theta > 0.0 && theta <= 1.0, PTE_ERR_ARG,
+            "bad oversample, margin or theta");
+    require(e->impl.cfg.precision == PTE_PRECISION_C128, PTE_ERR_ARG,
+            "the learned rank form runs with complex128 contractions");
+    RankState& r = e->impl.rank_state();
+    if (mode == 2) {
+      for (int i = 0; i < kRankParams; ++i)
+        require(std::isfinite(params[i]), PTE_ERR_NUMERIC, "non-finite MLP parameter");
+      if (!r.params) HIP_CHECK(hipMalloc((void**)&r.params, sizeof(float) * kRankParams));
+      HIP_CHECK(hipMemcpyAsync(r.params, params, sizeof(float) * kRankParams,
+                               hipMemcpyHostToDevice, e->impl.st()));
+      HIP_CHECK(hipStreamSynchronize(e->impl.st()));
+    }
+    r.mode = mode;
+    r.alpha = (float)alpha;
+    r.temperature = (float)temperature;
+    r.oversample = oversample;
+    r.margin = (float)margin;
+    r.theta = theta;
+  });
+}
+
+int pte_pt_rank_trace(pte_pt_engine* e, int capacity) {
+  return guarded([&] {
+    require(e != nullptr && capacity >= 0, PTE_ERR_ARG, "bad arguments");
+    RankState& r = e->impl.rank_state();
+    hipStream_t st = e->impl.st();
+    HIP_CHECK(hipStreamSynchronize(st));
+    if (r.rec) HIP_CHECK(hipFree(r.rec));
+    r.rec = nullptr;
+    r.rec_cap = capacity;
+    if (capacity > 0)
+      HIP_CHECK(hipMalloc((void**)&r.rec, sizeof(float) * (size_t)capacity * kRankRecord));
+    HIP_CHECK(hipMemsetAsync(r.rec_count, 0, sizeof(int), st));
+    HIP_CHECK(hipStreamSynchronize(st));
+  });
+}
+
+int pte_pt_get_rank_trace(pte_pt_engine* e, float* out, int max_records, int* count) {
+  return guarded([&] {
+    require(e != nullptr && count != nullptr, PTE_ERR_ARG, "null argument");
+    RankState* r = e->impl.ws.rank;
+    if (!r || !r->rec) {
+      *count = 0;
+      return;
+    }
+    hipStream_t st = e->impl.st();
+    int c = 0;
+    HIP_CHECK(hipMemcpyAsync(&c, r->rec_count, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    *count = c;
+    if (out && max_records > 0) {
+      const int n = std::min(c, max_records);
+      HIP_CHECK(hipMemcpyAsync(out, r->rec, sizeof(float) * (size_t)n * kRankRecord,
+                               hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int pte_pt_rank_stats(pte_pt_engine* e, long* svds, long* capped, long* fallbacks,
+                      long* qr_steps) {
+  return guarded([&] {
+    require(e && svds && capped && fallbacks && qr_steps, PTE_ERR_ARG, "null argument");
+    const RankState* r = e->impl.ws.rank;
+    *svds = r ? r->svds : 0;
+    *capped = r ? r->capped : 0;
+    *fallbacks = r ? r->fallbacks : 0;
+    *qr_steps = r ? r->qr_steps : 0;
+  });
+}
+
+// ---- the rank predictor's network on its own (training, inference) ----
+
+int pte_rank_param_count(void) { return kRankParams; }
+int pte_rank_feature_count(void) { return kRankFeatures; }
+
+int pte_rank_mlp_predict(const float* params, const float* X, int n, float* yhat) {
+  return guarded([&] {
+    require(params && X && yhat && n >= 1, PTE_ERR_ARG, "bad arguments");
+    MlpBuffers b;
+    float* P = b.upload(params, kRankParams);
+    float* dX = b.upload(X, (size_t)n * kRankFeatures);
+    float* H = b.alloc<float>((size_t)n * 3 * kRankWidth);
+    float* Y = b.alloc<float>(n);
+    mlp_forward_kernel<<<n, kRankWidth, 0, b.st>>>(P, dX, nullptr, n, H, Y);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(yhat, Y, sizeof(float) * n, hipMemcpyDeviceToHost, b.st));
+    HIP_CHECK(hipStreamSynchronize(b.st));
+  });
+}
+
+int pte_rank_mlp_grad(const float* params, const float* X, const float* y, int n, float* grad,
+                      float* loss) {
+  return guarded([&] {
+    require(params && X && y && grad && loss && n >= 1, PTE_ERR_ARG, "bad arguments");
+    MlpBuffers b;
+    float* P = b.upload(params, kRankParams);
+    float* dX = b.upload(X, (size_t)n * kRankFeatures);
+    float* dy = b.upload(y, n);
+    float* H = b.alloc<float>((size_t)n * 3 * kRankWidth);
+    float* D = b.alloc<float>((size_t)n * 3 * kRankWidth);
+    float* Y = b.alloc<float>(n);
+    float* d4 = b.alloc<float>(n);
+    float* G = b.alloc<float>(kRankParams);
+    float* L = b.alloc<float>(1);
+    mlp_forward_kernel<<<n, kRankWidth, 0, b.st>>>(P, dX, nullptr, n, H, Y);
+    mlp_backward_kernel<<<n, kRankWidth, 0, b.st>>>(P, H, Y, dy, nullptr, n, D, d4);
+    mlp_grad_kernel<<<(kRankParams + 255) / 256, 256, 0, b.st>>>(dX, nullptr, H, D, d4, n, G);
+    mlp_loss_kernel<<<1, 256, 0, b.st>>>(Y, dy, nullptr, n, L);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(grad, G, sizeof(float) * kRankParams, hipMemcpyDeviceToHost,
+                             b.st));
+    HIP_CHECK(hipMemcpyAsync(loss, L, sizeof(float), hipMemcpyDeviceToHost, b.st));
+    HIP_CHECK(hipStreamSynchronize(b.st));
+  });
+}
+
+int pte_rank_mlp_train(float* params, const float* X, const float* y, int n, int epochs,
+                       int batch, float lr, unsigned seed, float* epoch_loss) {
+  return guarded([&] {
+    require(params && X && y && n >= 1 && epochs >= 1 && batch >= 1 && lr > 0.f,
+            PTE_ERR_ARG, "bad arguments");
+    for (long i = 0; i < (long)n * kRankFeatures; ++i)
+      require(std::isfinite(X[i]), PTE_ERR_NUMERIC, "non-finite feature");
+    for (int i = 0; i < n; ++i)
+      require(std::isfinite(y[i]), PTE_ERR_NUMERIC, "non-finite target");
+    batch = std::min(batch, n);
+    MlpBuffers b;
+    float* P = b.upload(params, kRankParams);
+    float* dX = b.upload(X, (size_t)n * kRankFeatures);
+    float* dy = b.upload(y, n);
+    float* H = b.alloc<float>((size_t)batch * 3 * kRankWidth);
+    float* D = b.alloc<float>((size_t)batch * 3 * kRankWidth);
+    float* Y = b.alloc<float>(batch);
+    float* d4 = b.alloc<float>(batch);
+    float* G = b.alloc<float>(kRankParams);
+    float* M = b.alloc<float>(kRankParams);
+    float* V = b.alloc<float>(kRankParams);
+    const int nb = (n + batch - 1) / batch;
+    float* L = b.alloc<float>((size_t)epochs * nb);
+    int* idx = b.alloc<int>(n);
+    HIP_CHECK(hipMemsetAsync(M, 0, sizeof(float) * kRankParams, b.st));
+    HIP_CHECK(hipMemsetAsync(V, 0, sizeof(float) * kRankParams, b.st));
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    unsigned long long state = seed * 6364136223846793005ull + 1442695040888963407ull;
+    auto next = [&]() {   // 64-bit LCG (Knuth's MMIX constants), high bits
+      state = state * 6364136223846793005ull + 1442695040888963407ull;
+      return (unsigned)(state >> 33);
+    };
+    int t = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+      for (int i = n - 1; i > 0; --i) std::swap(order[i], order[next() % (unsigned)(i + 1)]);
+      // the host order is copied before the next epoch reshuffles it
+      HIP_CHECK(hipMemcpyAsync(idx, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                               b.st));
+      HIP_CHECK(hipStreamSynchronize(b.st));
+      for (int bi = 0; bi < nb; ++bi) {
+        const int off = bi * batch, cnt = std::min(batch, n - off);
+        mlp_forward_kernel<<<cnt, kRankWidth, 0, b.st>>>(P, dX, idx + off, cnt, H, Y);
+        mlp_loss_kernel<<<1, 256, 0, b.st>>>(Y, dy, idx + off, cnt, L + (long)ep * nb + bi);
+        mlp_backward_kernel<<<cnt, kRankWidth, 0, b.st>>>(P, H, Y, dy, idx + off, cnt, D, d4);
+        mlp_grad_kernel<<<(kRankParams + 255) / 256, 256, 0, b.st>>>(dX, idx + off, H, D, d4,
+                                                                    cnt, G);
+        ++t;
+        mlp_adam_kernel<<<(kRankParams + 255) / 256, 256, 0, b.st>>>(
+            P, G, M, V, kRankParams, lr, 0.9f, 0.999f, 1e-8f, t);
+      }
+      HIP_CHECK(hipGetLastError());
+    }
+    std::vector<float> lh((size_t)epochs * nb);
+    HIP_CHECK(hipMemcpyAsync(lh.data(), L, sizeof(float) * lh.size(), hipMemcpyDeviceToHost,
+                             b.st));
+    HIP_CHECK(hipMemcpyAsync(params, P, sizeof(float) * kRankParams, hipMemcpyDeviceToHost,
+                             b.st));
+    HIP_CHECK(hipStreamSynchronize(b.st));
+    if (epoch_loss)
+      for (int ep = 0; ep < epochs; ++ep) {
+        double acc = 0.0, w = 0.0;
+        for (int bi = 0; bi < nb; ++bi) {
+          const int cnt = std::min(batch, n - bi * batch);
+          acc += (double)lh[(size_t)ep * nb + bi] * cnt;
+          w += cnt;
+        }
+        epoch_loss[ep] = (float)(acc / w);
+      }
   });
 }
 

@@ -83,6 +83,16 @@ struct QrcpArgs {
   QrcpCand* cand;             // [2 * gridDim.x] scratch
   int* rank;                  // steps taken
   unsigned* bar;
+  const int* max_steps;       // optional cap on the steps (device int)
+  double* out;                // optional: [0] squared residual norm at the stop,
+                              // [1] ||A||_F^2 (first run), [2] |beta_0|, [3] the
+                              // first step count whose residual norm is at most
+                              // mark_rel * |beta_0| (the steps taken if none)
+  double mark_rel;            // 0: no mark
+  int stop_at_mark;           // stop at the mark (adaptive cap)
+  int start;                  // > 0: resume a factorisation stopped after `start`
+                              // steps (done, piv, tau, vscale, beta kept)
+  double drop_abs2;           // stop threshold on the squared residual when resuming
 };
 
 struct ApplyQArgs {
@@ -156,7 +166,7 @@ __global__ void jacobi_batch_kernel(const JacobiArgs* probs, const int* starts,
 __global__ void column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma);
 __global__ void rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted);
 __global__ void truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
-                                int max_rank);
+                                int max_rank, const double* extra2);
 __global__ void emit_kernel(EmitArgs a);
 __global__ void conj_copy_kernel(const double2* src, long count, double2* dst);
 __global__ void conj_transpose_kernel(const double2* src, int R, int C, double2* dst);

@@ -753,11 +753,15 @@ rank_sort_kernel(const double* sigma, int n, int* perm, double* sorted) {
   rank_sort_body(sigma, n, perm, sorted, blockIdx.x * 256 + threadIdx.x);
 }
 
+// extra2: squared norm of a part of the matrix left out of `sorted` (the
+// residual of a pivoted QR stopped at a rank cap), counted in every tail
 __device__ __forceinline__ void truncate_body(const double* sorted, int n, double epsrel,
-                                              SvdInfo* info, int max_rank, SvdInfo* copy) {
+                                              SvdInfo* info, int max_rank, SvdInfo* copy,
+                                              const double* extra2 = nullptr) {
   __shared__ double part[1024];
   __shared__ int cnt[1024];
   const int tid = threadIdx.x;
+  const double base2 = extra2 ? *extra2 : 0.0;
   const int per = (n + 1023) / 1024;
   const int lo = min(n, tid * per), hi = min(n, lo + per);
   double s = 0.0;
@@ -766,7 +770,7 @@ __device__ __forceinline__ void truncate_body(const double* sorted, int n, doubl
   __syncthreads();
   // suffix sums of the per-thread partials (serial over 1024 values by thread 0)
   if (tid == 0) {
-    double acc = 0.0;
+    double acc = base2;
     for (int t = 1023; t >= 0; --t) {
       const double v = part[t];
       part[t] = acc;        // sum of everything after thread t's segment
@@ -794,7 +798,7 @@ __device__ __forceinline__ void truncate_body(const double* sorted, int n, doubl
     info->s0 = s0;
     info->overflow = (k > max_rank) ? 1 : 0;
     // discarded 2-norm
-    double d = 0.0;
+    double d = base2;
     for (int i = n - 1; i >= k; --i) d += sorted[i] * sorted[i];
     info->discarded = sqrt(d);
     if (copy) *copy = *info;
@@ -803,8 +807,8 @@ __device__ __forceinline__ void truncate_body(const double* sorted, int n, doubl
 
 __global__ void __launch_bounds__(1024)
 truncate_kernel(const double* sorted, int n, double epsrel, SvdInfo* info,
-                int max_rank) {
-  truncate_body(sorted, n, epsrel, info, max_rank, nullptr);
+                int max_rank, const double* extra2) {
+  truncate_body(sorted, n, epsrel, info, max_rank, nullptr, extra2);
 }
 
 // ---- the same post-processing for many SVDs at once (problem = blockIdx.y,
@@ -959,16 +963,25 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
   const int nloc = (n - blk + nblk - 1) / nblk;   // own columns blk, blk+nblk, ...
   unsigned epoch = 0;
 
-  // initial column norms (one wavefront per column)
+  // initial column norms (one wavefront per column); when resuming, the
+  // norms of the unfactored rows start..m-1 of the columns not yet pivoted
+  const int j0 = a.start;
   double wmax = -1.0, wsum = 0.0;
   int wcol = -1;
   for (int q = wave; q < nloc; q += 4) {
     const int c = blk + q * nblk;
+    if (j0 > 0 && ld_agent(a.done + c)) {
+      if (lane == 0) done[q] = 1;
+      continue;
+    }
     const double2* col = a.A + (long)c * lda;
     double part = 0.0;
-    for (int i = lane; i < m; i += 64) part += cabs2(col[i]);
+    for (int i = j0 + lane; i < m; i += 64) part += cabs2(ld_agent(col + i));
     const double nrm = sqrt(wave_sum(part));
-    if (lane == 0) { vn1[q] = nrm; vn2[q] = nrm; done[q] = 0; st_agent(a.done + c, 0); }
+    if (lane == 0) {
+      vn1[q] = nrm; vn2[q] = nrm; done[q] = 0;
+      if (j0 == 0) st_agent(a.done + c, 0);
+    }
     wsum += nrm * nrm;
     better(wmax, wcol, nrm, c);
   }
@@ -977,11 +990,22 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
   double total;
   int p;
   qrcp_reduce(a.cand, nblk, slot, slotv, slotc, parity, &p, &total);
-  const double drop2 = a.drop * a.drop * total;
+  const double total0 = total;
+  const double drop2 = j0 > 0 ? a.drop_abs2 : a.drop * a.drop * total;
   const double tol3z = 1.4901161193847656e-08;   // sqrt(eps), zlaqp2
-  int j = 0;
+  const int jmax = a.max_steps ? min(kmax, max(1, *a.max_steps)) : kmax;
+  int j = j0;
+  // mark: the first step count with residual^2 <= (mark_rel |beta_0|)^2;
+  // every workgroup reduces the same candidates in the same order, so the
+  // test (and a stop on it) is uniform over the grid
+  int mark = -1;
+  double mark2 = -1.0;
+  if (a.mark_rel > 0.0 && j0 > 0) {
+    const double b0 = a.mark_rel * fabs(a.beta[0]);
+    mark2 = b0 * b0;
+  }
   PHASE_INIT();
-  while (j < kmax && p >= 0 && total > drop2) {
+  while (j < jmax && p >= 0 && total > drop2) {
     // reflector j from the pivot column p (rows j..m-1), whole workgroup
     const double2* pc = a.A + (long)p * lda;
     double part = 0.0;
@@ -1068,6 +1092,16 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
                 &total);
     PHASE(12);
     ++j;
+    if (a.mark_rel > 0.0) {
+      if (j == 1) {
+        const double b0 = a.mark_rel * fabs(beta);
+        mark2 = b0 * b0;
+      }
+      if (mark < 0 && total <= mark2) {
+        mark = j;
+        if (a.stop_at_mark) break;
+      }
+    }
   }
   // remaining columns in ascending order after the pivots
   if (blk == 0 && tid == 0) {
@@ -1075,6 +1109,12 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
     for (int c = 0; c < n; ++c)
       if (!ld_agent(a.done + c)) a.piv[t++] = c;
     *a.rank = j;
+    if (a.out) {
+      a.out[0] = (p >= 0 && j < kmax) ? total : 0.0;
+      if (j0 == 0) a.out[1] = total0;
+      a.out[2] = j > 0 ? fabs(a.beta[0]) : 0.0;
+      a.out[3] = (double)(mark >= 0 ? mark : j);
+    }
   }
 }
 

@@ -96,11 +96,18 @@ class PtTempo(BaseAPIClass):
         dkmax = self._parameters.dkmax
         if dkmax is None:
             dkmax = self._num_steps
+        config = dict(self._backend_config)
+        if config.get("rank_predictor") is not None \
+                or config.get("rank_trace", 0):
+            # bath features of the rank predictor
+            config.setdefault("rank_bath", (
+                float(getattr(self._correlations, "alpha", 0.0)),
+                float(getattr(self._correlations, "temperature", 0.0))))
         self._backend_instance = PtTempoBackend(
             dimension=self._dimension, influence=self._influence,
             process_tensor=self._process_tensor, sum_north=sum_north,
             sum_west=sum_west, num_steps=self._num_steps, dkmax=dkmax,
-            epsrel=self._parameters.epsrel, config=self._backend_config,
+            epsrel=self._parameters.epsrel, config=config,
             degeneracy_maps=maps)
 
     def _influence(self, dk: int) -> ndarray:
