@@ -52,14 +52,43 @@ def influence_table(bath, params):
             for dk in range(DKMAX + 1)}
 
 
-def gpu_run(warmup, steps, table, precision="c128"):
+def train_rank_predictor():
+    """The rank predictor trained on the engine's own traces of PT-TEMPO runs
+    of two other baths (alpha 0.3, w_c 3, T 0 and alpha 0.2, w_c 4, T 0.5;
+    dkmax 20, epsrel 1e-7, 40 steps); returns (predictor, summary)."""
+    import numpy as np
+    import oqupy_amd as oqupy
+    from oqupy_amd import operators as op
+    from oqupy_amd.rank_predictor import RankPredictor
+    t0 = time.perf_counter()
+    records = []
+    for alpha, wc, temp in ((0.3, 3.0, 0.0), (0.2, 4.0, 0.5)):
+        corr = oqupy.PowerLawSD(alpha=alpha, zeta=1.0, cutoff=wc,
+                                cutoff_type="exponential", temperature=temp)
+        bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
+        params = oqupy.TempoParameters(dt=DT, dkmax=20, epsrel=EPSREL)
+        ptt = oqupy.PtTempo(bath, 0.0, 40 * DT + 1e-9, params,
+                            backend_config={"rank_trace": 100000,
+                                            "chi_cap": 8192})
+        ptt.compute()
+        records.append(ptt.backend.rank_trace())
+    records = np.concatenate(records)
+    pred = RankPredictor(seed=0)
+    losses = pred.fit(records, epochs=60, batch=256, lr=1e-3, seed=1)
+    return pred, {"records": int(len(records)),
+                  "loss_first_last": [float(losses[0]), float(losses[-1])],
+                  "seconds": round(time.perf_counter() - t0, 2)}
+
+
+def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None):
     import torch
     import oqupy_amd as oqupy
     bath, params, np = build_model()
+    cfg = {"backend": "mi355x", "precision": precision, "chi_cap": 8192}
+    if rank_predictor is not None:
+        cfg["rank_predictor"] = rank_predictor
     ptt = oqupy.PtTempo(bath, 0.0, NSTEPS * DT + 1e-9, params,
-                        backend_config={"backend": "mi355x",
-                                        "precision": precision,
-                                        "chi_cap": 8192})
+                        backend_config=cfg)
     assert ptt.num_steps == NSTEPS
     ptt._influence = lambda dk: table.get(dk)
     backend = ptt.backend
@@ -77,7 +106,8 @@ def gpu_run(warmup, steps, table, precision="c128"):
     after = backend.svd_kernel_stats()
     window = backend.step_stats[-steps:]
     svd = {k: after[k] - before[k] for k in after}
-    return elapsed, svd, window, backend.step
+    rank = backend.rank_stats() if rank_predictor is not None else None
+    return elapsed, svd, window, backend.step, rank
 
 
 def blas_threads():
@@ -317,6 +347,12 @@ def main():
     ap.add_argument("--precision", choices=("c128", "c64"), default="c128",
                     help="c64: contractions in complex64 on the matrix cores "
                          "(FP32 accumulation), SVD and cut in FP64")
+    ap.add_argument("--rank", choices=("exact", "mlp", "previous", "adaptive"),
+                    default="exact",
+                    help="learned truncation-rank form: cap every pivoted QR "
+                         "at the MLP's prediction (trained first on traces of "
+                         "two other baths), at the bond's previous step count, "
+                         "or stop it adaptively")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; a "
@@ -337,8 +373,13 @@ def main():
     table = influence_table(bath, params)
     precompute_s = time.perf_counter() - t0
 
-    elapsed, svd, window, last_step = gpu_run(args.warmup, args.steps, table,
-                                              args.precision)
+    predictor, training = None, None
+    if args.rank == "mlp":
+        predictor, training = train_rank_predictor()
+    elif args.rank != "exact":
+        predictor = args.rank
+    elapsed, svd, window, last_step, rank = gpu_run(
+        args.warmup, args.steps, table, args.precision, predictor)
     value = args.steps / elapsed
     svd_avg_ms = svd["ms"] / max(svd["count"], 1)
     achieved = svd["flops"] / max(svd["ms"], 1e-12) / 1e9   # TFLOP/s
@@ -382,10 +423,14 @@ def main():
                 sum(w[1] for w in window) / max(1, sum(w[0] for w in window)),
                 2),
             "influence_precompute_s": round(precompute_s, 2),
+            "rank_form": args.rank,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if rank is not None:
+        out["config"]["rank_stats_whole_run"] = rank
+        out["config"]["rank_predictor_training"] = training
     print(json.dumps(out))
 
 
