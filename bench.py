@@ -26,30 +26,64 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+# PT-TEMPO workloads: headline spin-boson (c2), two qubits with a collective
+# bath (c3, Liouville dimension 16), and the long-memory, tight-tolerance run
+# of the learned form (c5)
+PT_CONFIGS = {
+    "c2": dict(name="spin-boson PT-TEMPO", dim=2, alpha=0.5, cutoff=5.0,
+               temp=0.2, dt=0.05, dkmax=200, epsrel=1e-7, nsteps=1000),
+    "c3": dict(name="two-qubit collective-bath PT-TEMPO", dim=4, alpha=0.2,
+               cutoff=4.0, temp=0.5, dt=0.05, dkmax=150, epsrel=1e-7,
+               nsteps=600),
+    "c5": dict(name="spin-boson PT-TEMPO (long memory)", dim=2, alpha=0.6,
+               cutoff=5.0, temp=0.1, dt=0.025, dkmax=400, epsrel=1e-8,
+               nsteps=2000),
+}
+CFG = dict(PT_CONFIGS["c2"])
 ALPHA, CUTOFF, TEMP, DT, DKMAX, EPSREL, NSTEPS = 0.5, 5.0, 0.2, 0.05, 200, \
     1e-7, 1000
-FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def select_config(key):
+    """Make `key` the PT-TEMPO workload of this run."""
+    global ALPHA, CUTOFF, TEMP, DT, DKMAX, EPSREL, NSTEPS
+    CFG.clear()
+    CFG.update(PT_CONFIGS[key])
+    ALPHA, CUTOFF, TEMP, DT = CFG["alpha"], CFG["cutoff"], CFG["temp"], CFG["dt"]
+    DKMAX, EPSREL, NSTEPS = CFG["dkmax"], CFG["epsrel"], CFG["nsteps"]
+
+
+def coupling():
+    """S = sigma_z / 2 (one spin) or (sigma_z x 1 + 1 x sigma_z) / 2."""
+    import numpy as np
+    from oqupy_amd import operators as op
+    if CFG["dim"] == 2:
+        return 0.5 * op.sigma("z")
+    eye = np.eye(2)
+    return 0.5 * (np.kron(op.sigma("z"), eye) + np.kron(eye, op.sigma("z")))
 
 
 def build_model():
     import numpy as np
-    from oqupy_amd import operators as op
     from oqupy_amd.bath import Bath
     from oqupy_amd.bath_correlations import PowerLawSD
     from oqupy_amd.tempo import TempoParameters
     corr = PowerLawSD(alpha=ALPHA, zeta=1.0, cutoff=CUTOFF,
                       cutoff_type="exponential", temperature=TEMP)
-    bath = Bath(0.5 * op.sigma("z"), corr)
+    bath = Bath(coupling(), corr)
     params = TempoParameters(dt=DT, dkmax=DKMAX, epsrel=EPSREL)
     return bath, params, np
 
 
-def influence_table(bath, params):
+def influence_table(bath, params, count=None):
     from oqupy_amd.tempo import influence_matrix
     corr = bath.correlations
+    count = DKMAX + 1 if count is None else count
     return {dk: influence_matrix(dk, params, corr, bath.coupling_acomm,
                                  bath.coupling_comm)
-            for dk in range(DKMAX + 1)}
+            for dk in range(count)}
 
 
 def train_rank_predictor():
@@ -128,8 +162,9 @@ def cpu_baseline(warmup, steps, table, budget_s):
     work): steps/s over the steps it completed."""
     from oracle.pt_tempo import PtTempoOracle
     import numpy as np
-    pt = PtTempoOracle(2, lambda dk: table.get(dk), np.ones(4), NSTEPS, DKMAX,
-                       EPSREL)
+    d = CFG["dim"]
+    pt = PtTempoOracle(d, lambda dk: table.get(dk), np.ones(d * d), NSTEPS,
+                       DKMAX, EPSREL)
     pt.initialize()
     for _ in range(warmup):
         pt.compute_step()
@@ -338,9 +373,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of timed oracle work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c1"), default="c2",
-                    help="c1: the TEMPO configuration instead of the headline "
-                         "PT-TEMPO run")
+    ap.add_argument("--config", choices=("c2", "c1", "c3", "c5"), default="c2",
+                    help="c1: the TEMPO configuration; c3: two qubits with a "
+                         "collective bath (Liouville dimension 16); c5: the "
+                         "long-memory run (dt 0.025, dkmax 400, epsrel 1e-8); "
+                         "default c2, the headline PT-TEMPO run")
     ap.add_argument("--batch", type=int, default=0,
                     help="time the batched form on this many instances of the "
                          "seeded sweep of baths instead of the headline run")
@@ -368,8 +405,12 @@ def main():
         batch_main(args)
         return
 
+    if args.config in PT_CONFIGS:
+        select_config(args.config)
     bath, params, np = build_model()
     t0 = time.perf_counter()
+    # influence tensors dk = 0 .. dkmax (the window needs only the first ones
+    # plus the initial dkmax + 1 of the MPO)
     table = influence_table(bath, params)
     precompute_s = time.perf_counter() - t0
 
@@ -403,14 +444,14 @@ def main():
                          f"{cores}) on steps {args.warmup + 2}.."
                          f"{args.warmup + 1 + done} of the same workload"}
     out = {
-        "metric": "PT-TEMPO time-steps/s at dkmax=200",
+        "metric": f"PT-TEMPO time-steps/s at dkmax={DKMAX}",
         "value": round(value, 6), "unit": "time-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * elapsed / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic",
         "config": {
-            "workload": (f"spin-boson PT-TEMPO alpha={ALPHA} wc={CUTOFF} "
+            "workload": (f"{CFG['name']} alpha={ALPHA} wc={CUTOFF} "
                          f"T={TEMP} dt={DT} dkmax={DKMAX} epsrel={EPSREL} "
                          f"N={NSTEPS}; timed steps {args.warmup + 2}.."
                          f"{args.warmup + 1 + args.steps}"),

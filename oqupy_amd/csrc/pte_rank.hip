@@ -1,7 +1,8 @@
 // Learned truncation-rank form: device kernels of the rank predictor.
 //
-// A small MLP (24 features -> 3 x 128 tanh -> 1) predicts log2 of the number
-// of pivoted-QR steps a truncated SVD needs before its residual falls below
+// A small MLP (24 features -> 3 x 128 tanh -> 1) predicts the change, from the
+// bond's previous truncation, of log2 of the number of pivoted-QR steps a
+// truncated SVD needs before its residual falls below
 // the acceptance threshold (a fraction of epsrel times the largest column
 // norm), from the bond's previous truncation (its last 16 log singular values
 // and step count), the position of the bond, the step, the bath and the
@@ -85,7 +86,9 @@ rank_predict_kernel(RankPredictArgs a) {
   int cap = a.nmax;
   if (kp > 0) {
     if (a.params) {
-      const float y = mlp_forward_one(a.params, x, h, red);
+      // the network predicts the change of log2(steps) / 8 from the bond's
+      // previous truncation (x[19])
+      const float y = mlp_forward_one(a.params, x, h, red) + x[19];
       const float k = exp2f(fminf(8.0f * y, 20.0f));
       cap = (int)ceilf(k * (1.0f + a.margin)) + a.oversample;
     } else {

@@ -24,7 +24,10 @@ Features of one truncation (``pte_rank_feature_count()`` = 24): step / N,
 distance of the site from the newest one / dkmax, kind (0 zip-up, 1 sweep),
 the bond's last 16 kept singular values at its previous truncation as
 log10(s / s0) / 8, log2 of the steps it needed then / 8, alpha, temperature,
-log10(epsrel) / 10 and log2(min(m, n)) / 8. Target: log2(need) / 8.
+log10(epsrel) / 10 and log2(min(m, n)) / 8. Record target: log2(need) / 8; the
+network is trained on (and predicts) its change from the previous truncation,
+log2(need / previous need) / 8, which carries over to memory lengths and bond
+dimensions outside the training runs.
 """
 
 from typing import Optional
@@ -87,11 +90,14 @@ class RankPredictor:
 
     def fit(self, records: np.ndarray, epochs: int = 40, batch: int = 256,
             lr: float = 1e-3, seed: int = 0) -> np.ndarray:
-        """Train on trace records (Adam, mean squared error on log2(k)/8);
-        returns the mean loss of every epoch."""
+        """Train on trace records (Adam, mean squared error on the change of
+        log2(steps)/8 from the bond's previous truncation); returns the mean
+        loss of every epoch."""
         feats, target, _ = split_records(usable(records))
         if len(target) == 0:
             raise ValueError("no usable training records.")
+        # the network learns the change from the bond's previous step count
+        target = np.ascontiguousarray(target - feats[:, 19])
         eng.require_gpu()
         lib = eng.load_library()
         losses = np.zeros(epochs, dtype=np.float32)
@@ -105,7 +111,8 @@ class RankPredictor:
         return losses
 
     def predict(self, features: np.ndarray) -> np.ndarray:
-        """Network outputs (log2(k)/8) for rows of features, on the GPU."""
+        """Network outputs for rows of features, on the GPU: the predicted
+        change of log2(steps) / 8 from the bond's previous truncation."""
         feats = np.ascontiguousarray(features, dtype=np.float32)
         if feats.ndim != 2 or feats.shape[1] != FEATURES:
             raise ValueError(f"features must be (n, {FEATURES}).")
@@ -118,9 +125,12 @@ class RankPredictor:
                                                eng.fptr(out)))
         return out
 
-    def predict_rank(self, features: np.ndarray) -> np.ndarray:
-        """Predicted kept ranks (not rounded)."""
-        return np.exp2(8.0 * self.predict(features).astype(np.float64))
+    def predict_steps(self, features: np.ndarray) -> np.ndarray:
+        """Predicted pivoted-QR step counts (not rounded): the network's
+        change added to the bond's previous log2(steps) / 8."""
+        feats = np.asarray(features, dtype=np.float32)
+        out = self.predict(feats).astype(np.float64) + feats[:, 19]
+        return np.exp2(8.0 * out)
 
     def gradient(self, features: np.ndarray, targets: np.ndarray):
         """(loss, gradient) of the mean squared error, on the GPU."""
