@@ -9,6 +9,8 @@ correlations, systems and influence tensors are host-side FP64 code.
 from oqupy_amd import operators
 from oqupy_amd.bath import Bath
 from oqupy_amd.control import Control
+from oqupy_amd.correlations import (compute_correlations,
+                                    compute_correlations_nt)
 from oqupy_amd.bath_correlations import (CustomCorrelations, CustomSD,
                                          PowerLawSD)
 from oqupy_amd.dynamics import Dynamics
@@ -24,7 +26,8 @@ from oqupy_amd.version import __version__
 __all__ = [
     'Bath', 'Control', 'CustomCorrelations', 'CustomSD', 'Dynamics', 'PowerLawSD',
     'PtTempo', 'SimpleProcessTensor', 'System', 'Tempo', 'TempoParameters',
-    'TimeDependentSystem', 'TrivialProcessTensor', 'compute_dynamics',
+    'TimeDependentSystem', 'TrivialProcessTensor', 'compute_correlations',
+    'compute_correlations_nt', 'compute_dynamics',
     'compute_dynamics_batch',
     'influence_matrix', 'operators', 'pt_tempo_compute',
     'pt_tempo_compute_batch', 'tempo_compute',
