@@ -208,6 +208,28 @@ def c4_table(i):
             for dk in range(k + 1)}
 
 
+def c4_tables_device(count):
+    """Influence tables of the first `count` instances from eta evaluated on
+    the GPU, all baths in one launch (the host QUADPACK path takes 10-30 s
+    per instance)."""
+    from oqupy_amd import operators as op
+    from oqupy_amd.bath import Bath
+    from oqupy_amd.bath_correlations import PowerLawSD
+    from oqupy_amd.influence import device_influences
+    from oqupy_amd.tempo import TempoParameters
+    baths, params, dkmaxes = [], [], []
+    for i in range(count):
+        a, wc, t, k = c4_instance(i)
+        corr = PowerLawSD(alpha=a, zeta=1.0, cutoff=wc,
+                          cutoff_type="exponential", temperature=t)
+        baths.append(Bath(0.5 * op.sigma("z"), corr))
+        params.append(TempoParameters(dt=C4_DT, dkmax=k, epsrel=C4_EPSREL))
+        dkmaxes.append(k)
+    infls = device_influences(baths, params, dkmaxes)
+    return [{dk: inf(dk) for dk in range(k + 1)}
+            for inf, k in zip(infls, dkmaxes)]
+
+
 def c4_engines(count, tables):
     import oqupy_amd as oqupy
     from oqupy_amd import operators as op
@@ -239,21 +261,34 @@ def batch_main(args):
     from oqupy_amd.backends import engine as eng
     count = args.batch
     t0 = time.perf_counter()
-    with ProcessPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
-        tables = list(ex.map(c4_table, range(count)))
+    if args.host_influence:
+        with ProcessPoolExecutor(max(1, min(16, os.cpu_count() or 1))) as ex:
+            tables = list(ex.map(c4_table, range(count)))
+    else:
+        tables = c4_tables_device(count)
     precompute_s = time.perf_counter() - t0
+    print(f"influence tables of {count} instances: {precompute_s:.2f} s",
+          file=sys.stderr, flush=True)
     lib = eng.load_library()
+
+    def note(msg):
+        print(f"[batch {count}] {msg} ({time.perf_counter() - t0:.1f} s)",
+              file=sys.stderr, flush=True)
     backends = c4_engines(count, tables)
+    note("engines initialised")
     handles = (ctypes.c_void_p * count)(*[b.handle.value for b in backends])
     batch = ctypes.c_void_p()
     eng.check(lib.pte_pt_batch_create(handles, count, ctypes.byref(batch)))
     more = np.ones(count, dtype=np.int32)
     eng.check(lib.pte_pt_batch_run(batch, 1, eng.iptr(more)))
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.check(lib.pte_pt_batch_run(batch, args.steps, eng.iptr(more)))
+    note("batched warm-up step done")
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.check(lib.pte_pt_batch_run(batch, 1, eng.iptr(more)))
+        note("batched step done")
     torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
+    t_batch = time.perf_counter() - t1
     counts = [ctypes.c_long() for _ in range(3)]
     times = [ctypes.c_double() for _ in range(3)]
     eng.check(lib.pte_pt_batch_stats(batch, *[ctypes.byref(c) for c in counts],
@@ -267,12 +302,14 @@ def batch_main(args):
     for b in single:
         b.compute_step()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    note("one-by-one engines ready")
+    t1 = time.perf_counter()
     for _ in range(args.steps):
         for b in single:
             b.compute_step()
+        note("one-by-one step done")
     torch.cuda.synchronize()
-    t_single = time.perf_counter() - t0
+    t_single = time.perf_counter() - t1
     value = count * args.steps / t_batch
     print(json.dumps({
         "metric": "batched PT-TEMPO instance-steps/s (sum over instances)",
@@ -296,7 +333,9 @@ def batch_main(args):
                    "serve_s": round(times[0].value / 1e3, 3),
                    "batched_qr_s": round(times[1].value / 1e3, 3),
                    "batched_jacobi_s": round(times[2].value / 1e3, 3),
-                   "influence_precompute_s": round(precompute_s, 2)},
+                   "influence_precompute_s": round(precompute_s, 2),
+                   "influence": "host QUADPACK" if args.host_influence
+                                else "eta on the GPU (pte_eta_batch)"},
         "roofline": None, "cpu_baseline": None}))
 
 
@@ -381,6 +420,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="time the batched form on this many instances of the "
                          "seeded sweep of baths instead of the headline run")
+    ap.add_argument("--host-influence", action="store_true",
+                    help="--batch: influence tables from host QUADPACK eta "
+                         "instead of the GPU eta kernel")
     ap.add_argument("--precision", choices=("c128", "c64"), default="c128",
                     help="c64: contractions in complex64 on the matrix cores "
                          "(FP32 accumulation), SVD and cut in FP64")

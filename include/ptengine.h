@@ -296,6 +296,17 @@ int pte_rank_mlp_grad(const float* params, const float* X, const float* y, int n
 int pte_rank_mlp_train(float* params, const float* X, const float* y, int n, int epochs,
                        int batch, float lr, unsigned seed, float* epoch_loss);
 
+/* eta(tau) of power-law baths for many times in one launch: bath b has
+ * bath_params[4b..4b+3] = alpha, zeta (>= 1), w_c, T and cutoff_types[b]
+ * (0 exponential, 1 Gaussian, 2 hard); time i belongs to bath tau_bath[i];
+ * eta_out receives n_tau complex values. gl_nodes / gl_weights: the 16-point
+ * Gauss-Legendre rule on [-1, 1]. Replaces the QUADPACK integrals of
+ * CustomSD.eta_function (oqupy/bath_correlations.py:518-591) that every
+ * influence tensor (oqupy/tempo.py:969-1020) is built from. */
+int pte_eta_batch(int n_baths, const double* bath_params, const int* cutoff_types,
+                  const double* gl_nodes, const double* gl_weights, int n_tau,
+                  const int* tau_bath, const double* taus, double* eta_out);
+
 const char* pte_last_error(void);
 
 /* Library version string. */

@@ -125,6 +125,8 @@ _SIGNATURES = {
     "pte_rank_mlp_train": (ctypes.c_int, [_FP, _FP, _FP, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_float, ctypes.c_uint,
                                           _FP]),
+    "pte_eta_batch": (ctypes.c_int, [ctypes.c_int, _DP, _IP, _DP, _DP,
+                                     ctypes.c_int, _IP, _DP, _DP]),
     "pte_dynamics_run_multi": (ctypes.c_int, [ctypes.c_int] * 3 + [_IP, _DP, _DP, _DP,
                                               ctypes.POINTER(_DP),
                                               ctypes.POINTER(_DP), _DP, _DP]),
@@ -242,6 +244,10 @@ def parse_backend_config(config: Optional[dict]) -> dict:
         raise ValueError("rank_oversample >= 0, rank_margin >= 0 and "
                          "0 < rank_theta <= 1 are required.")
     bath = tuple(float(v) for v in cfg.get("rank_bath", (0.0, 0.0)))
+    influence = cfg.get("influence", "host")
+    if influence not in ("host", "device"):
+        raise ValueError("influence must be 'host' or 'device'.")
+    out["influence"] = influence
     out.update(rank_predictor=predictor, rank_trace=trace,
                rank_oversample=oversample, rank_margin=margin,
                rank_theta=theta, rank_bath=bath)

@@ -11,7 +11,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["pte_kernels.hip", "pte_rank.hip", "pte_engine.hip"]
+SOURCES = ["pte_kernels.hip", "pte_rank.hip", "pte_eta.hip", "pte_engine.hip"]
 TARGET = os.path.join(HERE, "libptengine.so")
 
 

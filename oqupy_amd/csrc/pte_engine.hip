@@ -28,6 +28,7 @@
 #include "pte_device.h"
 #include "pte_kernels.h"
 #include "pte_rank.h"
+#include "pte_eta.h"
 
 namespace pte {
 
@@ -2645,6 +2646,54 @@ int pte_pt_rank_stats(pte_pt_engine* e, long* svds, long* capped, long* fallback
 }
 
 // ---- the rank predictor's network on its own (training, inference) ----
+
+int pte_eta_batch(int n_baths, const double* bath_params, const int* cutoff_types,
+                  const double* gl_nodes, const double* gl_weights, int n_tau,
+                  const int* tau_bath, const double* taus, double* eta_out) {
+  return guarded([&] {
+    require(n_baths >= 1 && n_tau >= 1 && bath_params && cutoff_types && gl_nodes &&
+                gl_weights && tau_bath && taus && eta_out,
+            PTE_ERR_ARG, "bad arguments");
+    std::vector<EtaBath> baths(n_baths);
+    std::vector<double> tmax(n_baths, 0.0);
+    for (int i = 0; i < n_tau; ++i) {
+      require(tau_bath[i] >= 0 && tau_bath[i] < n_baths, PTE_ERR_ARG, "bad bath index");
+      require(std::isfinite(taus[i]), PTE_ERR_ARG, "non-finite time");
+      tmax[tau_bath[i]] = std::max(tmax[tau_bath[i]], std::fabs(taus[i]));
+    }
+    for (int b = 0; b < n_baths; ++b) {
+      EtaBath& e = baths[b];
+      e.alpha = bath_params[4 * b];
+      e.zeta = bath_params[4 * b + 1];
+      e.wc = bath_params[4 * b + 2];
+      e.temp = bath_params[4 * b + 3];
+      e.cutoff = cutoff_types[b];
+      require(e.cutoff >= 0 && e.cutoff <= 2, PTE_ERR_ARG, "unknown cutoff type");
+      require(std::isfinite(e.alpha) && e.zeta >= 1.0 && e.wc > 0.0 && e.temp >= 0.0,
+              PTE_ERR_ARG, "bath parameters: zeta >= 1, w_c > 0, T >= 0 required");
+      const double top = e.wc * (e.cutoff == ETA_CUTOFF_EXPONENTIAL ? 40.0
+                                 : e.cutoff == ETA_CUTOFF_GAUSSIAN ? 6.5 : 1.0);
+      double width = e.wc / 8.0;
+      if (tmax[b] > 0.0) width = std::min(width, 1.0 / tmax[b]);
+      const double np = std::ceil(top / width);
+      require(np <= 1e7, PTE_ERR_CAPACITY, "time range too long for the eta quadrature");
+      e.npan = std::max(1, (int)np);
+      e.width = top / e.npan;
+    }
+    MlpBuffers buf;   // device buffers freed on exit
+    EtaBath* db = buf.upload(baths.data(), baths.size());
+    int* dtb = buf.upload(tau_bath, n_tau);
+    double* dt = buf.upload(taus, n_tau);
+    double* gx = buf.upload(gl_nodes, kEtaNodes);
+    double* gw = buf.upload(gl_weights, kEtaNodes);
+    double2* out = buf.alloc<double2>(n_tau);
+    eta_kernel<<<n_tau, 256, 0, buf.st>>>(db, dtb, dt, n_tau, gx, gw, out);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(eta_out, out, sizeof(double2) * n_tau, hipMemcpyDeviceToHost,
+                             buf.st));
+    HIP_CHECK(hipStreamSynchronize(buf.st));
+  });
+}
 
 int pte_rank_param_count(void) { return kRankParams; }
 int pte_rank_feature_count(void) { return kRankFeatures; }

@@ -13,6 +13,7 @@ from typing import Dict, List, Optional, Text, Union
 import numpy as np
 from numpy import ndarray
 
+from oqupy_amd.backends.engine import parse_backend_config as eng_config
 from oqupy_amd.base_api import BaseAPIClass
 from oqupy_amd.bath import Bath
 from oqupy_amd.config import PT_TEMPO_BACKEND_CONFIG
@@ -65,6 +66,11 @@ class PtTempo(BaseAPIClass):
             "Parameter `end_time` must be more than two times steps larger "
             "than the parameter `start_time`!")
         self._num_steps = num_steps
+        # influence tensors from eta evaluated on the device (built at first
+        # use, or handed in by pt_tempo_compute_batch for many baths at once)
+        self._device_influence = None
+        self._use_device_influence = \
+            eng_config(self._backend_config)["influence"] == "device"
         self._backend_instance = None
         self._init_pt_tempo_backend()
 
@@ -111,12 +117,25 @@ class PtTempo(BaseAPIClass):
             degeneracy_maps=maps)
 
     def _influence(self, dk: int) -> ndarray:
+        if self._use_device_influence:
+            if self._device_influence is None:
+                self._device_influence = self._make_device_influence()
+            return self._device_influence(dk)
         positions = degeneracy_positions(self._bath) if self._unique else None
         return influence_matrix(dk, parameters=self._parameters,
                                 correlations=self._correlations,
                                 coupling_acomm=self._bath.coupling_acomm,
                                 coupling_comm=self._bath.coupling_comm,
                                 deg_positions=positions)
+
+    def _make_device_influence(self, eta=None):
+        from oqupy_amd.influence import DeviceInfluence
+        dkmax = self._parameters.dkmax
+        if dkmax is None:
+            dkmax = self._num_steps
+        positions = degeneracy_positions(self._bath) if self._unique else None
+        return DeviceInfluence(self._bath, self._parameters, dkmax, positions,
+                               eta)
 
     @property
     def dimension(self) -> int:
@@ -220,6 +239,16 @@ def pt_tempo_compute_batch(baths: List[Bath],
                 "Batched PT-TEMPO does not support `add_correlation_time`.")
     ptts = [PtTempo(b, s, e, p, unique, None, False, backend_config, n)
             for b, s, e, p, n in zip(baths, starts, ends, params, names)]
+    if ptts[0]._use_device_influence:
+        # eta of every bath on its time grid in one device launch
+        from oqupy_amd.influence import eta_values
+        dkmaxes = [p.dkmax if p.dkmax is not None else t.num_steps
+                   for p, t in zip(params, ptts)]
+        etas = eta_values([b.correlations for b in baths],
+                          [np.arange(k + 2) * p.dt
+                           for p, k in zip(params, dkmaxes)])
+        for ptt, eta in zip(ptts, etas):
+            ptt._device_influence = ptt._make_device_influence(eta)
     backends = [p.backend for p in ptts]
     for b in backends:
         b.initialize()
