@@ -27,6 +27,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
+JACOBI_HBM_BYTES_PER_LAUNCH = 62.8e6
+JACOBI_TRAFFIC_SOURCE = ("profiles/hbm_traffic_jacobi_qrcp_pt_tempo_dkmax200_"
+                         "steps2-10.md")
 
 # PT-TEMPO workloads: headline spin-boson (c2), two qubits with a collective
 # bath (c3, Liouville dimension 16), and the long-memory, tight-tolerance run
@@ -432,7 +435,21 @@ def main():
                          "at the MLP's prediction (trained first on traces of "
                          "two other baths), at the bond's previous step count, "
                          "or stop it adaptively")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 4 times (the "
+                         "plain copy kernel that calibrates the HBM byte "
+                         "counters of a profile) and exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n, "launches": 4}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; a "
                           "single PT-TEMPO instance is a sequential chain of "
@@ -475,6 +492,12 @@ def main():
         "launches": svd["count"], "avg_launch_ms": round(svd_avg_ms, 4),
         "algorithmic_flops_per_launch": svd["flops"] / max(svd["count"], 1),
     }
+    if args.config == "c2" and args.precision == "c128" and args.rank == "exact":
+        # HBM bytes per jacobi_kernel launch of this workload (steps 2-10),
+        # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a
+        # plain copy (fetch x2.0, write x1.0)
+        roofline["traffic"] = JACOBI_HBM_BYTES_PER_LAUNCH
+        roofline["traffic_source"] = JACOBI_TRAFFIC_SOURCE
     cpu = None
     if not args.no_cpu:
         cores = blas_threads()
