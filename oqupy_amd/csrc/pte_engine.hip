@@ -543,7 +543,16 @@ struct Workspace {
   // (mode 3 stops there without a prediction).
   int svd_qrcp(int m, int n, int key = -1) {
     if (batcher) return batched_svd(true, m, n);
-    if (small_ok(m, n)) return svd_small(m, n);
+    if (small_ok(m, n)) {
+      const bool adaptive = rank != nullptr && key >= 0 && rank->mode == 3;
+      const int k = svd_small(m, n, adaptive);
+      if (adaptive) {
+        rank->svds += 1;
+        rank->capped += 1;
+        rank->qr_steps += info_h->pad;
+      }
+      return k;
+    }
     const bool learn = rank != nullptr && key >= 0;
     const bool capped = learn && rank->mode > 0;
     const RankQuery q = learn ? rank->query(key, m, n, (float)epsrel) : RankQuery();
@@ -666,6 +675,7 @@ struct Workspace {
     sa.sigma = sigma; sa.perm = perm; sa.sorted = sorted; sa.info = info_d;
     sa.epsrel = epsrel; sa.max_rank = chi_cap; sa.max_sweeps = max_sweeps;
     sa.copy = copy;
+    sa.stop_rel = 0.0;
     return sa;
   }
 
@@ -680,8 +690,11 @@ struct Workspace {
     return jacobi_check(n, r, ms, true);
   }
 
-  int svd_small(int m, int n) {
+  // `adaptive`: the learned form's adaptive stop of the QR (its residual
+  // counted in the cut)
+  int svd_small(int m, int n, bool adaptive = false) {
     SmallSvdArgs sa = small_prepare(m, n, nullptr);
+    if (adaptive) sa.stop_rel = rank->theta * epsrel;
     HIP_CHECK(hipEventRecord(evj0, svd_stream));
     small_svd_kernel<<<1, 256, small_lds(m, n), svd_stream>>>(sa);
     HIP_CHECK(hipEventRecord(evj1, svd_stream));

@@ -134,6 +134,7 @@ struct SmallSvdArgs {
   double* sigma; int* perm; double* sorted; SvdInfo* info;
   double epsrel; int max_rank; int max_sweeps;
   SvdInfo* copy;    // optional second copy of the summary (batched gather)
+  double stop_rel;  // > 0: stop the QR at residual <= stop_rel * |beta_0|
 };
 
 constexpr int kSmallMaxCols = 64;           // columns of M in small_svd_kernel

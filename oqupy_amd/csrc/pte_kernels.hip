@@ -1233,6 +1233,9 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   const double tol3z = 1.4901161193847656e-08;
   const int kmax = m < n ? m : n;
   int j = 0;
+  // adaptive stop (stop_rel > 0): once the residual is at most
+  // stop_rel * |beta_0|; the residual is then counted in the cut
+  double stop2 = -1.0, folded2 = 0.0;
   while (j < kmax && p >= 0 && total > drop2) {
     const double2* pc = As + (long)p * m;
     // every wavefront sums the pivot column itself (no workgroup barrier)
@@ -1310,6 +1313,14 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     __syncthreads();
     choose();
     ++j;
+    if (a.stop_rel > 0.0) {
+      if (j == 1) stop2 = (a.stop_rel * beta) * (a.stop_rel * beta);
+      // `total` is computed identically by every wavefront: a uniform exit
+      if (total <= stop2) {
+        folded2 = (p >= 0 && j < kmax) ? total : 0.0;
+        break;
+      }
+    }
   }
   const int r = j;
   PHASE(13);
@@ -1456,13 +1467,13 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     // smallest k with sqrt(sum_{i>=k} s_i^2) <= epsrel s_0 (suffix sums)
     const double s0 = s_sorted[0];
     const double thr = a.epsrel * s0;
-    double tail = 0.0;
+    double tail = folded2;
     int k = 0;
     for (int i = r - 1; i >= 0; --i) {
       tail += s_sorted[i] * s_sorted[i];
       if (sqrt(tail) > thr) { k = i + 1; break; }
     }
-    double d = 0.0;
+    double d = folded2;
     for (int i = r - 1; i >= k; --i) d += s_sorted[i] * s_sorted[i];
     SvdInfo inf;
     inf.rank = k;
