@@ -2394,8 +2394,9 @@ static int guarded(F&& f) {
   }
 }
 
-// device buffers of one call of the rank predictor's network, freed on exit
-struct MlpBuffers {
+// device buffers and stream of one stateless C-ABI call (the rank
+// predictor's network, eta), freed on exit
+struct CallBuffers {
   hipStream_t st = nullptr;
   std::vector<void*> owned;
   template <class T>
@@ -2411,8 +2412,8 @@ struct MlpBuffers {
     HIP_CHECK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
     return p;
   }
-  MlpBuffers() { HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
-  ~MlpBuffers() {
+  CallBuffers() { HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
+  ~CallBuffers() {
     (void)hipStreamSynchronize(st);
     for (void* p : owned) (void)hipFreeAsync(p, st);
     (void)hipStreamSynchronize(st);
@@ -2693,7 +2694,7 @@ int pte_eta_batch(int n_baths, const double* bath_params, const int* cutoff_type
       e.npan = std::max(1, (int)np);
       e.width = top / e.npan;
     }
-    MlpBuffers buf;   // device buffers freed on exit
+    CallBuffers buf;   // device buffers freed on exit
     EtaBath* db = buf.upload(baths.data(), baths.size());
     int* dtb = buf.upload(tau_bath, n_tau);
     double* dt = buf.upload(taus, n_tau);
@@ -2714,7 +2715,7 @@ int pte_rank_feature_count(void) { return kRankFeatures; }
 int pte_rank_mlp_predict(const float* params, const float* X, int n, float* yhat) {
   return guarded([&] {
     require(params && X && yhat && n >= 1, PTE_ERR_ARG, "bad arguments");
-    MlpBuffers b;
+    CallBuffers b;
     float* P = b.upload(params, kRankParams);
     float* dX = b.upload(X, (size_t)n * kRankFeatures);
     float* H = b.alloc<float>((size_t)n * 3 * kRankWidth);
@@ -2730,7 +2731,7 @@ int pte_rank_mlp_grad(const float* params, const float* X, const float* y, int n
                       float* loss) {
   return guarded([&] {
     require(params && X && y && grad && loss && n >= 1, PTE_ERR_ARG, "bad arguments");
-    MlpBuffers b;
+    CallBuffers b;
     float* P = b.upload(params, kRankParams);
     float* dX = b.upload(X, (size_t)n * kRankFeatures);
     float* dy = b.upload(y, n);
@@ -2762,7 +2763,7 @@ int pte_rank_mlp_train(float* params, const float* X, const float* y, int n, int
     for (int i = 0; i < n; ++i)
       require(std::isfinite(y[i]), PTE_ERR_NUMERIC, "non-finite target");
     batch = std::min(batch, n);
-    MlpBuffers b;
+    CallBuffers b;
     float* P = b.upload(params, kRankParams);
     float* dX = b.upload(X, (size_t)n * kRankFeatures);
     float* dy = b.upload(y, n);
