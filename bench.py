@@ -415,6 +415,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0,
                     help="seconds of timed oracle work for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip timing the adaptive rank form beside the "
+                         "plain one")
     ap.add_argument("--config", choices=("c2", "c1", "c3", "c5"), default="c2",
                     help="c1: the TEMPO configuration; c3: two qubits with a "
                          "collective bath (Liouville dimension 16); c5: the "
@@ -537,6 +540,19 @@ def main():
     if rank is not None:
         out["config"]["rank_stats_whole_run"] = rank
         out["config"]["rank_predictor_training"] = training
+    if args.config == "c2" and args.rank == "exact" and not args.no_forms:
+        # the learned form's adaptive rank cap on the same window, beside the
+        # plain form (not the metric: its results differ from the reference's
+        # by the truncation's own sensitivity, see the learned-form tests)
+        try:
+            el, _, _, _, rs = gpu_run(args.warmup, args.steps, table,
+                                      args.precision, "adaptive")
+            out["other_forms"] = {"adaptive_rank_cap": {
+                "value": round(args.steps / el, 6), "unit": "time-steps/s",
+                "speedup_vs_plain": round(elapsed / el, 3),
+                "rank_stats_whole_run": rs}}
+        except Exception as err:   # the plain form's line is still printed
+            out["other_forms"] = {"adaptive_rank_cap": {"error": str(err)}}
     print(json.dumps(out))
 
 
