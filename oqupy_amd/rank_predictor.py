@@ -9,7 +9,7 @@ norm, a lower bound of s0), the QR is stopped at the prediction times
 ``1 + rank_margin`` plus ``rank_oversample`` steps, and the Jacobi SVD runs on
 the smaller factor; the residual then counts as discarded weight in the
 epsrel cut. When the capped factorisation's residual is above the threshold
-the QR resumes where it stopped and runs to the exact stop, which also judges
+the QR resumes where it stopped and runs to the exact stop, which also checks
 the predictor.
 
 The engine records training data while it runs (``backend_config=
