@@ -1,5 +1,5 @@
 """Benchmark: PT-TEMPO time-steps/s of the spin-boson process tensor at
-dkmax = 200 on one MI355X.
+dkmax = 200 on one MI355X, and max |d<sigma_z>| against the FP64 oracle.
 
 Workload (the headline configuration): H_S = (Omega/2) sigma_x with Omega = 1,
 coupling sigma_z/2, Ohmic J(w) = 2 alpha w exp(-w/w_c) with alpha = 0.5,
@@ -7,13 +7,18 @@ w_c = 5, T = 0.2; dt = 0.05, dkmax = 200, epsrel = 1e-7, 1000 steps,
 complex128. A "step" is one PT-TEMPO ``compute_step`` (zip-up of the
 201-node influence MPO into the device-resident MPS plus the truncating
 sweep). The influence tensors are computed on the host before the timed
-region; ``--warmup`` steps run untimed, then ``--steps`` steps are timed
-between device synchronisations. The bond dimensions of this workload grow
-by about 11 per step during the first ~200 steps, so the timed window is the
-early part of the run; the JSON line names it.
+region. The bond dimension of this workload grows by ~12 per step for the
+first ~200 steps, so the run first goes untimed through steps 2..41 (max bond
+~500 at step 40), then ``--warmup`` untimed steps, then ``--steps`` timed
+steps between device synchronisations; the JSON line names the window.
 
-The CPU baseline times the FP64 oracle (numpy/LAPACK) on the same window of
-the same workload on the host cores.
+The CPU baseline is the FP64 oracle (numpy/LAPACK) continuing from the
+engine's MPS at the window start, with the BLAS threads set to the host's
+cores and to one, for a bounded time; its rate is the share of the window's
+first step (in algorithmic SVD flops, from the engine's own SVD log of that
+step) it completes per second. The accuracy half of the metric comes from a
+whole run the oracle finishes in seconds: the 200-step TEMPO workload
+(dkmax 40, epsrel 1e-6), max |d<sigma_z>| over all 201 times.
 """
 
 import argparse
@@ -27,9 +32,18 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 FP64_MFMA_PEAK_TFLOPS = 78.6
-JACOBI_HBM_BYTES_PER_LAUNCH = 62.8e6
-JACOBI_TRAFFIC_SOURCE = ("profiles/hbm_traffic_jacobi_qrcp_pt_tempo_dkmax200_"
-                         "steps2-10.md")
+
+# untimed steps that take each workload into its large-bond phase before the
+# warm-up and the timed window (the headline run's bond dimension grows by
+# ~12 per step and passes 500 at step 40)
+PHASE_STEPS = {"c2": 40, "c3": 0, "c5": 0}
+
+# HBM bytes per launch of the roofline kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of the same command and window, scaled by
+# what the counters read for a plain copy (keyed by config, precision, rank
+# form, phase steps, warm-up, timed steps)
+TRAFFIC = {}
+TRAFFIC_SOURCE = None
 
 # PT-TEMPO workloads: headline spin-boson (c2), two qubits with a collective
 # bath (c3, Liouville dimension 16), and the long-memory, tight-tolerance run
@@ -105,8 +119,7 @@ def train_rank_predictor():
         bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
         params = oqupy.TempoParameters(dt=DT, dkmax=20, epsrel=EPSREL)
         ptt = oqupy.PtTempo(bath, 0.0, 40 * DT + 1e-9, params,
-                            backend_config={"rank_trace": 100000,
-                                            "chi_cap": 8192})
+                            backend_config={"rank_trace": 100000})
         ptt.compute()
         records.append(ptt.backend.rank_trace())
     records = np.concatenate(records)
@@ -117,11 +130,25 @@ def train_rank_predictor():
                   "seconds": round(time.perf_counter() - t0, 2)}
 
 
-def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None):
+def svd_flops(m, n):
+    """Algorithmic flops of a thin complex SVD of an m x n matrix: the
+    R-SVD count of Golub & Van Loan, 4 (6 M N^2 + 20 N^3) with M = max and
+    N = min (complex arithmetic = 4 real)."""
+    big, small = max(m, n), min(m, n)
+    return 4.0 * (6.0 * big * small * small + 20.0 * small ** 3)
+
+
+def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None,
+            export_state=False):
+    """The engine on the workload: initialise, `warmup` untimed steps, then
+    `steps` timed steps. Returns a dict: elapsed seconds, Jacobi kernel
+    statistics of the window, per-step statistics, the SVD log of the window
+    (m, n, kept spectrum per SVD) and, with export_state, the MPS sites at
+    the window start (the CPU leg continues from them)."""
     import torch
     import oqupy_amd as oqupy
     bath, params, np = build_model()
-    cfg = {"backend": "mi355x", "precision": precision, "chi_cap": 8192}
+    cfg = {"backend": "mi355x", "precision": precision}
     if rank_predictor is not None:
         cfg["rank_predictor"] = rank_predictor
     ptt = oqupy.PtTempo(bath, 0.0, NSTEPS * DT + 1e-9, params,
@@ -131,9 +158,20 @@ def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None):
     backend = ptt.backend
     backend._influence = ptt._influence
     backend.initialize()
-    for _ in range(warmup):
+    for i in range(warmup):
         backend.compute_step()
+        if (i + 1) % 10 == 0:
+            print(f"[bench] step {backend.step} max bond "
+                  f"{backend.step_stats[-1][2]}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    state = None
+    if export_state:
+        n_sites = len(backend.bond_dimensions()) + 1
+        state = (backend.step, [backend.get_site(i) for i in range(n_sites)])
+    # the SVD log (shape and kept spectrum of every truncated SVD) gives the
+    # algorithmic work of the window; it costs one small device-to-host copy
+    # per SVD
+    backend.svd_log(200_000_000)
     before = backend.svd_kernel_stats()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -144,7 +182,11 @@ def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None):
     window = backend.step_stats[-steps:]
     svd = {k: after[k] - before[k] for k in after}
     rank = backend.rank_stats() if rank_predictor is not None else None
-    return elapsed, svd, window, backend.step, rank
+    log = backend.get_svd_log()
+    backend.release()
+    return {"elapsed": elapsed, "svd": svd, "window": window,
+            "last_step": backend.step, "rank": rank, "log": log,
+            "state": state}
 
 
 def blas_threads():
@@ -160,25 +202,50 @@ def blas_threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
 
 
-def cpu_baseline(warmup, steps, table, budget_s):
-    """The oracle on the same window (capped at ``budget_s`` seconds of timed
-    work): steps/s over the steps it completed."""
+class _BudgetSpent(Exception):
+    pass
+
+
+class _Deadline(list):
+    """The oracle's SVD log that ends its run once the time budget is spent
+    (checked after every completed SVD)."""
+
+    def __init__(self, seconds):
+        super().__init__()
+        self.t_end = time.perf_counter() + seconds
+
+    def append(self, item):
+        super().append(item)
+        if time.perf_counter() > self.t_end:
+            raise _BudgetSpent()
+
+
+def cpu_baseline(state, table, step_flops, budget_s, threads):
+    """The FP64 oracle continuing from the engine's state at the window
+    start (the same MPS sites), with the BLAS/LAPACK library limited to
+    `threads` threads, for about `budget_s` seconds of the window's first
+    step. Its rate in steps/s is the share of that step's algorithmic SVD
+    flops completed (per SVD, from the engine's log of the same step) per
+    second. Returns (steps/s, SVDs completed, flop share completed)."""
+    from threadpoolctl import threadpool_limits
     from oracle.pt_tempo import PtTempoOracle
     import numpy as np
     d = CFG["dim"]
+    step, sites = state
     pt = PtTempoOracle(d, lambda dk: table.get(dk), np.ones(d * d), NSTEPS,
                        DKMAX, EPSREL)
-    pt.initialize()
-    for _ in range(warmup):
-        pt.compute_step()
-    t0 = time.perf_counter()
-    done = 0
-    for _ in range(steps):
-        pt.compute_step()
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    return done / (time.perf_counter() - t0), done
+    pt.load_state(step, sites)
+    with threadpool_limits(limits=threads, user_api="blas"):
+        pt.spectra = _Deadline(budget_s)
+        t0 = time.perf_counter()
+        try:
+            pt.compute_step()
+        except _BudgetSpent:
+            pass
+        elapsed = time.perf_counter() - t0
+    done = sum(svd_flops(m, n) for m, n, _ in pt.spectra)
+    share = done / step_flops
+    return share / elapsed, len(pt.spectra), share
 
 
 C4_SEED, C4_N, C4_DT, C4_EPSREL = 0, 500, 0.05, 1e-7
@@ -244,7 +311,7 @@ def c4_engines(count, tables):
         bath = oqupy.Bath(0.5 * op.sigma("z"), corr)
         params = oqupy.TempoParameters(dt=C4_DT, dkmax=k, epsrel=C4_EPSREL)
         ptt = oqupy.PtTempo(bath, 0.0, C4_N * C4_DT + 1e-9, params,
-                            backend_config={"chi_cap": 8192})
+                            backend_config={})
         tab = tables[i]
         ptt.backend._influence = lambda dk, tab=tab: tab.get(dk)
         ptt.backend.initialize()
@@ -342,12 +409,13 @@ def batch_main(args):
         "roofline": None, "cpu_baseline": None}))
 
 
-def tempo_main(args):
-    """TEMPO on the spin-boson configuration with a short memory: H_S =
-    sigma_x / 2, coupling sigma_z / 2, Ohmic alpha 0.3, w_c 3, T 0,
-    rho_0 = |up><up|, dt 0.05, dkmax 40, epsrel 1e-6, 200 steps; the whole
-    step loop is timed (influences precomputed), and the FP64 oracle runs the
-    same steps on the host cores for the baseline and the accuracy."""
+def tempo_c1(precision="c128", cpu=True):
+    """TEMPO on the short-memory spin-boson workload (H_S = sigma_x / 2,
+    coupling sigma_z / 2, Ohmic alpha 0.3, w_c 3, T 0, rho_0 = |up><up|,
+    dt 0.05, dkmax 40, epsrel 1e-6, 200 steps) on the engine and, with
+    `cpu`, on the FP64 oracle with the same influences. Returns (GPU
+    seconds for steps 2..200, oracle seconds for the same steps or None,
+    max |d<sigma_z>| over all 201 times or None)."""
     import numpy as np
     import torch
     import oqupy_amd as oqupy
@@ -365,7 +433,7 @@ def tempo_main(args):
                                   bath.coupling_acomm, bath.coupling_comm)
              for dk in range(dkmax + 1)}
     tempo = oqupy.Tempo(system, bath, params, rho0, 0.0,
-                        backend_config={"precision": args.precision})
+                        backend_config={"precision": precision})
     tempo._backend_instance._influence = lambda dk: table.get(dk)
     tempo.compute(end_time=0.05 * 1 + 1e-9)          # initialise + step 1
     torch.cuda.synchronize()
@@ -373,7 +441,9 @@ def tempo_main(args):
     dyn = tempo.compute(end_time=0.05 * n + 1e-9)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    steps = n - 1
+    tempo._backend_instance.release()
+    if not cpu:
+        return elapsed, None, None
     props = system.get_propagators(0.05, 0.0, params.subdiv_limit,
                                    params.liouvillian_epsrel)
     ref = TempoOracle(rho0, lambda dk: table.get(dk), bath.unitary_transform,
@@ -383,13 +453,34 @@ def tempo_main(args):
     _, st = ref.compute_step()
     states.append(st.reshape(2, 2))
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(n - 1):
         _, st = ref.compute_step()
         states.append(st.reshape(2, 2))
     cpu_elapsed = time.perf_counter() - t0
     sz = op.sigma("z")
     dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
                  - np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
+    return elapsed, cpu_elapsed, float(dsz.max())
+
+
+def tempo_accuracy():
+    """The accuracy half of the metric on a whole run that the oracle
+    finishes in seconds: max |d<sigma_z>| of the 200-step TEMPO workload
+    against the FP64 oracle."""
+    elapsed, cpu_elapsed, dsz = tempo_c1()
+    return {"workload": "spin-boson TEMPO alpha=0.3 wc=3.0 T=0 dt=0.05 "
+                        "dkmax=40 epsrel=1e-6 N=200 (whole run)",
+            "max_abs_dsigma_z": dsz,
+            "gpu_steps_per_s": round(199 / elapsed, 3),
+            "oracle_steps_per_s": round(199 / cpu_elapsed, 3)}
+
+
+def tempo_main(args):
+    """The TEMPO workload on its own line: the whole step loop timed
+    (influences precomputed), the FP64 oracle on the same steps on the host
+    cores for the baseline and the accuracy."""
+    steps = 199
+    elapsed, cpu_elapsed, dsz = tempo_c1(args.precision)
     print(json.dumps({
         "metric": "TEMPO time-steps/s (dkmax=40); max |d<sigma_z>| vs FP64 oracle",
         "value": round(steps / elapsed, 4), "unit": "time-steps/s",
@@ -399,7 +490,7 @@ def tempo_main(args):
         "dtype": args.precision, "data": "synthetic",
         "config": {"workload": "spin-boson TEMPO alpha=0.3 wc=3.0 T=0 dt=0.05 "
                                "dkmax=40 epsrel=1e-6 N=200, steps 2..200",
-                   "max_abs_dsigma_z": float(dsz.max())},
+                   "max_abs_dsigma_z": dsz},
         "roofline": None,
         "cpu_baseline": {"value": round(steps / cpu_elapsed, 4),
                          "unit": "time-steps/s", "cores": blas_threads(),
@@ -412,12 +503,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-budget", type=float, default=25.0,
-                    help="seconds of timed oracle work for the CPU baseline")
+    ap.add_argument("--phase-steps", type=int, default=-1,
+                    help="untimed steps before the warm-up that take the run "
+                         "into its large-bond phase (default: 40 for c2, 0 "
+                         "otherwise)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of oracle work for the CPU baseline (half "
+                         "of it again for the one-thread leg)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="BLAS threads of the CPU baseline (default: the "
+                         "host's cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-forms", action="store_true",
-                    help="skip timing the adaptive rank form beside the "
-                         "plain one")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the whole-run TEMPO accuracy check")
     ap.add_argument("--config", choices=("c2", "c1", "c3", "c5"), default="c2",
                     help="c1: the TEMPO configuration; c3: two qubits with a "
                          "collective bath (Liouville dimension 16); c5: the "
@@ -471,19 +569,31 @@ def main():
         select_config(args.config)
     bath, params, np = build_model()
     t0 = time.perf_counter()
-    # influence tensors dk = 0 .. dkmax (the window needs only the first ones
-    # plus the initial dkmax + 1 of the MPO)
+    # influence tensors dk = 0 .. dkmax (the MPO), computed before the timed
+    # region like the reference's own timing (its progress timer starts after
+    # initialize)
     table = influence_table(bath, params)
     precompute_s = time.perf_counter() - t0
+    phase = args.phase_steps if args.phase_steps >= 0 else PHASE_STEPS[
+        args.config]
+    warm = phase + args.warmup
+    first = warm + 2
 
     predictor, training = None, None
     if args.rank == "mlp":
         predictor, training = train_rank_predictor()
     elif args.rank != "exact":
         predictor = args.rank
-    elapsed, svd, window, last_step, rank = gpu_run(
-        args.warmup, args.steps, table, args.precision, predictor)
+    run = gpu_run(warm, args.steps, table, args.precision, predictor,
+                  export_state=not args.no_cpu)
+    elapsed, svd, window, log = run["elapsed"], run["svd"], run["window"], \
+        run["log"]
     value = args.steps / elapsed
+    per_step = [w[0] for w in window]
+    first_step_log = log[:per_step[0]]
+    step_flops = [sum(svd_flops(m, n) for m, n, _ in
+                      log[sum(per_step[:i]):sum(per_step[:i + 1])])
+                  for i in range(len(per_step))]
     svd_avg_ms = svd["ms"] / max(svd["count"], 1)
     achieved = svd["flops"] / max(svd["ms"], 1e-12) / 1e9   # TFLOP/s
     roofline = {
@@ -491,26 +601,34 @@ def main():
         "bound": "mfma", "achieved": round(achieved, 4),
         "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 6),
-        "traffic": None,
+        "traffic": TRAFFIC.get((args.config, args.precision, args.rank,
+                                phase, args.warmup, args.steps)),
         "launches": svd["count"], "avg_launch_ms": round(svd_avg_ms, 4),
         "algorithmic_flops_per_launch": svd["flops"] / max(svd["count"], 1),
+        "timing": "HIP events around every launch on the engine's stream",
     }
-    if args.config == "c2" and args.precision == "c128" and args.rank == "exact":
-        # HBM bytes per jacobi_kernel launch of this workload (steps 2-10),
-        # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by a
-        # plain copy (fetch x2.0, write x1.0)
-        roofline["traffic"] = JACOBI_HBM_BYTES_PER_LAUNCH
-        roofline["traffic_source"] = JACOBI_TRAFFIC_SOURCE
+    if roofline["traffic"] is not None:
+        roofline["traffic_source"] = TRAFFIC_SOURCE
     cpu = None
     if not args.no_cpu:
-        cores = blas_threads()
-        cpu_value, done = cpu_baseline(args.warmup, args.steps, table,
-                                       args.cpu_budget)
-        cpu = {"value": round(cpu_value, 6), "unit": "time-steps/s",
+        cores = os.cpu_count() if args.cpu_threads <= 0 else args.cpu_threads
+        cores = min(cores, 16)
+        rate, n_done, share = cpu_baseline(run["state"], table, step_flops[0],
+                                           args.cpu_budget, cores)
+        rate1, n_done1, share1 = cpu_baseline(run["state"], table,
+                                              step_flops[0],
+                                              args.cpu_budget / 2, 1)
+        cpu = {"value": round(rate, 6), "unit": "time-steps/s",
                "cores": cores, "kind": "port",
-               "sample": f"FP64 oracle (numpy/LAPACK, OpenBLAS threads = "
-                         f"{cores}) on steps {args.warmup + 2}.."
-                         f"{args.warmup + 1 + done} of the same workload"}
+               "sample": (f"FP64 oracle (numpy/LAPACK zgesdd) continuing "
+                          f"from the engine's MPS at the window start, step "
+                          f"{first}: its first {n_done} truncated SVDs "
+                          f"({100 * share:.1f} % of that step's algorithmic "
+                          f"SVD flops) in ~{args.cpu_budget:.0f} s; steps/s "
+                          f"= flop share / seconds"),
+               "one_thread": {"value": round(rate1, 6), "cores": 1,
+                              "svds": n_done1,
+                              "flop_share": round(share1, 4)}}
     out = {
         "metric": f"PT-TEMPO time-steps/s at dkmax={DKMAX}",
         "value": round(value, 6), "unit": "time-steps/s",
@@ -521,38 +639,34 @@ def main():
         "config": {
             "workload": (f"{CFG['name']} alpha={ALPHA} wc={CUTOFF} "
                          f"T={TEMP} dt={DT} dkmax={DKMAX} epsrel={EPSREL} "
-                         f"N={NSTEPS}; timed steps {args.warmup + 2}.."
-                         f"{args.warmup + 1 + args.steps}"),
+                         f"N={NSTEPS}; steps 2..{phase + 1} untimed to reach "
+                         f"the large-bond phase, {args.warmup} untimed "
+                         f"warm-up steps, timed steps {first}.."
+                         f"{first + args.steps - 1}"),
             "precision": ("plain complex128" if args.precision == "c128" else
                           "complex64 contractions (FP32 accumulation), "
                           "FP64 SVD and cut"),
+            "max_bond_at_window_start": int(max(
+                max(1, len(sp)) for _, _, sp in first_step_log)),
             "max_bond_in_window": max(w[2] for w in window),
-            "svds_per_step": round(sum(w[0] for w in window) / len(window), 1),
+            "svds_per_step": round(sum(per_step) / len(per_step), 1),
+            "largest_svd_in_window": max(((m, n) for m, n, _ in log),
+                                         key=lambda x: x[0] * x[1]),
+            "algorithmic_svd_tflop_per_step": round(
+                sum(step_flops) / len(step_flops) / 1e12, 3),
             "jacobi_sweeps_per_svd": round(
-                sum(w[1] for w in window) / max(1, sum(w[0] for w in window)),
-                2),
+                sum(w[1] for w in window) / max(1, sum(per_step)), 2),
             "influence_precompute_s": round(precompute_s, 2),
             "rank_form": args.rank,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    if rank is not None:
-        out["config"]["rank_stats_whole_run"] = rank
+    if run["rank"] is not None:
+        out["config"]["rank_stats_whole_run"] = run["rank"]
         out["config"]["rank_predictor_training"] = training
-    if args.config == "c2" and args.rank == "exact" and not args.no_forms:
-        # the learned form's adaptive rank cap on the same window, beside the
-        # plain form (not the metric: its results differ from the reference's
-        # by the truncation's own sensitivity, see the learned-form tests)
-        try:
-            el, _, _, _, rs = gpu_run(args.warmup, args.steps, table,
-                                      args.precision, "adaptive")
-            out["other_forms"] = {"adaptive_rank_cap": {
-                "value": round(args.steps / el, 6), "unit": "time-steps/s",
-                "speedup_vs_plain": round(elapsed / el, 3),
-                "rank_stats_whole_run": rs}}
-        except Exception as err:   # the plain form's line is still printed
-            out["other_forms"] = {"adaptive_rank_cap": {"error": str(err)}}
+    if args.config == "c2" and not args.no_accuracy:
+        out["accuracy"] = tempo_accuracy()
     print(json.dumps(out))
 
 

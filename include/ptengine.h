@@ -46,7 +46,8 @@ typedef struct {
   int dkmax;        /* memory cut K (the backend's dkmax, never None)         */
   double epsrel;    /* relative SVD truncation error                          */
   int precision;    /* PTE_PRECISION_C128 or PTE_PRECISION_C64                */
-  int chi_cap;      /* capacity of every bond; exceeding it is an error       */
+  int chi_cap;      /* limit of every bond (INT_MAX: none, the default of the
+                       Python side); a truncation above it is an error      */
   int n_north;      /* length of sum_north (d^2, or the degeneracy classes)   */
 } pte_pt_config;
 
@@ -90,6 +91,13 @@ int pte_pt_bond_dims(pte_pt_engine* e, int* out, int n);
 int pte_pt_get_mpo_tensor(pte_pt_engine* e, int step, double* out, long cap,
                           int* shape3);
 
+/* MPS site i of the current step as stored, [left][physical][right] (no
+ * scaling), at any step after initialisation; shape3 receives its shape.
+ * Exposes the NodeArray node the reference keeps for that site
+ * (oqupy/backends/node_array.py:45, pt_tempo_backend.py:166-169): it hands a
+ * computation's state to a checker mid-run. */
+int pte_pt_get_site(pte_pt_engine* e, int i, double* out, long cap, int* shape3);
+
 /* Caps of the process tensor, computed on the device: caps[k] has length
  * bond_past of step k (caps[N] = [1]). Replaces SimpleProcessTensor.compute_caps
  * (oqupy/process_tensor.py:380-406). pte_pt_get_cap copies caps[k]. */
@@ -108,6 +116,21 @@ int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps,
  * roofline report of the benchmark. */
 int pte_pt_svd_kernel_stats(pte_pt_engine* e, long* count, double* ms,
                             double* flops, double* bytes);
+
+/* Log of the truncated SVDs that follow (no counterpart in the reference: it
+ * exposes what split_node_full_svd keeps at oqupy/backends/node_array.py:262,
+ * 285, 541): per SVD the shape of the matrix the reference factors (rows,
+ * columns), the kept rank and the kept singular values, descending.
+ * Recording stops once max_values singular values are held; max_values = 0
+ * turns the log off. Each call clears the log. */
+int pte_pt_svd_log(pte_pt_engine* e, long max_values);
+
+/* Copy the log: shapes receives (m, n, k) per SVD for up to max_svds SVDs,
+ * values the kept singular values of those SVDs concatenated, up to
+ * max_values; *n_svds and *n_values receive the totals held (NULL buffers:
+ * counts only). */
+int pte_pt_get_svd_log(pte_pt_engine* e, int* shapes, int max_svds, double* values,
+                       long max_values, int* n_svds, long* n_values);
 
 void pte_pt_destroy(pte_pt_engine* e);
 

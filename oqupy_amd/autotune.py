@@ -61,7 +61,7 @@ def _window(bath, params, n_total, warmup, steps, table):
     import torch
     import oqupy_amd as oqupy
     ptt = oqupy.PtTempo(bath, 0.0, n_total * params.dt + 1e-9, params,
-                        backend_config={"chi_cap": 8192})
+                        backend_config={})
     backend = ptt.backend
     backend._influence = lambda dk: table.get(dk)
     backend.initialize()

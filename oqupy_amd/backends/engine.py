@@ -8,14 +8,23 @@ bad arguments, ``AssertionError`` for calls out of order, ``RuntimeError``
 for device, capacity and numerical failures.
 """
 
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 from typing import Optional
 
 import numpy as np
 
 _LIB_NAME = "libptengine.so"
 _LIB = None
+
+# Backends that hold a live engine handle. Their engines are destroyed by an
+# exit hook that runs before the interpreter finalises and before the HIP
+# runtime (and any profiler attached to it) is torn down; a handle released
+# any later would call hipFree / hipStreamDestroy on a finalised runtime.
+_LIVE = weakref.WeakSet()
 
 PTE_OK = 0
 PTE_ERR_ARG = -1
@@ -74,12 +83,16 @@ _SIGNATURES = {
     "pte_pt_bond_dims": (ctypes.c_int, [_P, _IP, ctypes.c_int]),
     "pte_pt_get_mpo_tensor": (ctypes.c_int, [_P, ctypes.c_int, _DP,
                                              ctypes.c_long, _IP]),
+    "pte_pt_get_site": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_long, _IP]),
     "pte_pt_compute_caps": (ctypes.c_int, [_P]),
     "pte_pt_get_cap": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int,
                                       _IP]),
     "pte_pt_last_step_stats": (ctypes.c_int, [_P, _IP, _IP, _IP, _DP]),
     "pte_pt_svd_kernel_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_long),
                                                _DP, _DP, _DP]),
+    "pte_pt_svd_log": (ctypes.c_int, [_P, ctypes.c_long]),
+    "pte_pt_get_svd_log": (ctypes.c_int, [_P, _IP, ctypes.c_int, _DP, ctypes.c_long,
+                                          _IP, ctypes.POINTER(ctypes.c_long)]),
     "pte_pt_destroy": (None, [_P]),
     "pte_tempo_create": (ctypes.c_int, [ctypes.POINTER(_TempoConfig),
                                         ctypes.POINTER(_P)]),
@@ -160,7 +173,27 @@ def load_library(path: Optional[str] = None):
         fn.argtypes = args
     if path is None:
         _LIB = lib
+        # registered after torch's own exit hooks, so it runs before them
+        atexit.register(release_all)
     return lib
+
+
+def track(backend) -> None:
+    """Record a backend that now owns an engine handle (see release_all)."""
+    _LIVE.add(backend)
+
+
+def release_all() -> None:
+    """Destroy every live engine (the exit hook; also safe to call early)."""
+    for backend in list(_LIVE):
+        backend.release()
+    _LIVE.clear()
+
+
+def finalizing() -> bool:
+    """True while the interpreter shuts down: engine handles still alive then
+    are left to the process exit instead of calling into the runtime."""
+    return sys.is_finalizing()
 
 
 def check(code: int) -> None:
@@ -222,9 +255,11 @@ def parse_backend_config(config: Optional[dict]) -> dict:
     if precision not in PRECISIONS:
         raise ValueError(f"Unsupported precision '{precision}'; use 'c128' "
                          "or 'c64'.")
-    chi_cap = int(cfg.get("chi_cap", DEFAULT_CHI_CAP))
-    if chi_cap < 1:
-        raise ValueError("chi_cap must be positive.")
+    chi_cap = cfg.get("chi_cap", None)
+    chi_cap = DEFAULT_CHI_CAP if chi_cap is None else int(chi_cap)
+    if chi_cap < 1 or chi_cap > DEFAULT_CHI_CAP:
+        raise ValueError("chi_cap must be a positive C int (or None for "
+                         "no limit).")
     out = {"backend": backend, "precision": precision, "chi_cap": chi_cap}
     # learned truncation-rank form (PT-TEMPO only)
     predictor = cfg.get("rank_predictor", None)
@@ -244,6 +279,10 @@ def parse_backend_config(config: Optional[dict]) -> dict:
         raise ValueError("rank_oversample >= 0, rank_margin >= 0 and "
                          "0 < rank_theta <= 1 are required.")
     bath = tuple(float(v) for v in cfg.get("rank_bath", (0.0, 0.0)))
+    svd_log = int(cfg.get("svd_log", 0))
+    if svd_log < 0:
+        raise ValueError("svd_log must be >= 0.")
+    out["svd_log"] = svd_log
     influence = cfg.get("influence", "host")
     if influence not in ("host", "device"):
         raise ValueError("influence must be 'host' or 'device'.")

@@ -11,6 +11,7 @@ it. The tensor chain stays in device memory.
 """
 
 import ctypes
+import weakref
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
@@ -28,7 +29,10 @@ class PtTempoBackend:
                  degeneracy_maps: Optional[List[ndarray]] = None) -> None:
         self._dimension = dimension
         self._influence = influence
-        self._process_tensor = process_tensor
+        # held weakly: the finished process tensor keeps this backend (its
+        # device-resident engine) alive, and a strong reference back would
+        # make a cycle that only the garbage collector could free
+        self._process_tensor_ref = weakref.ref(process_tensor)
         self._sum_north = np.asarray(sum_north)
         self._sum_west = np.asarray(sum_west)
         self._num_steps = num_steps
@@ -69,6 +73,7 @@ class PtTempoBackend:
         eng.check(self._lib.pte_pt_create(ctypes.byref(cfg),
                                           ctypes.byref(handle)))
         self._handle = handle
+        eng.track(self)
 
     def initialize(self) -> None:
         """Build the MPO and MPS and sweep them (pt_tempo_backend.py:105-190)."""
@@ -95,6 +100,8 @@ class PtTempoBackend:
         pivoted QRs (the MLP's prediction, the bond's previous step count, or
         the adaptive stop at the acceptance threshold)."""
         cfg = self._config
+        if cfg["svd_log"] > 0:
+            self.svd_log(cfg["svd_log"])
         if cfg["rank_trace"] > 0:
             eng.check(self._lib.pte_pt_rank_trace(self._handle,
                                                   cfg["rank_trace"]))
@@ -162,6 +169,31 @@ class PtTempoBackend:
         self.step_stats.append(tuple(s.value for s in stats))
         return bool(more.value)
 
+    def svd_log(self, max_values: int) -> None:
+        """Record the truncated SVDs that follow (up to ``max_values`` kept
+        singular values in total; 0 turns the log off)."""
+        eng.check(self._lib.pte_pt_svd_log(self._handle, int(max_values)))
+
+    def get_svd_log(self) -> List[tuple]:
+        """The recorded SVDs in order: ``(m, n, kept singular values)``, the
+        shape of the matrix the reference factors and its kept spectrum."""
+        ns, nv = ctypes.c_int(), ctypes.c_long()
+        eng.check(self._lib.pte_pt_get_svd_log(self._handle, None, 0, None, 0,
+                                               ctypes.byref(ns),
+                                               ctypes.byref(nv)))
+        shapes = np.zeros(3 * max(ns.value, 1), dtype=np.int32)
+        values = np.zeros(max(nv.value, 1), dtype=np.float64)
+        eng.check(self._lib.pte_pt_get_svd_log(
+            self._handle, eng.iptr(shapes), ns.value,
+            values.ctypes.data_as(eng._DP), nv.value, ctypes.byref(ns),
+            ctypes.byref(nv)))
+        out, off = [], 0
+        for i in range(ns.value):
+            m, n, k = (int(v) for v in shapes[3 * i:3 * i + 3])
+            out.append((m, n, values[off:off + k].copy()))
+            off += k
+        return out
+
     def sync_step(self) -> None:
         """Read the step counter back from the engine (after batched steps)."""
         st = ctypes.c_int()
@@ -187,6 +219,24 @@ class PtTempoBackend:
         eng.check(self._lib.pte_pt_bond_dims(self._handle, eng.iptr(out),
                                              int(out.size)))
         return [int(b) for b in out]
+
+    def get_site(self, index: int) -> ndarray:
+        """MPS site ``index`` as stored on the device, ``[left, physical,
+        right]`` (any step after initialisation)."""
+        n = ctypes.c_int()
+        eng.check(self._lib.pte_pt_num_sites(self._handle, ctypes.byref(n)))
+        bonds = [1] + self.bond_dimensions() + [1]
+        if not 0 <= index < n.value:
+            raise IndexError("site index out of range")
+        # the physical leg is d^2 (1 for a site appended this step)
+        count = bonds[index] * bonds[index + 1] * self._dimension ** 2
+        out = np.empty(count, dtype=np.complex128)
+        shape = np.zeros(3, dtype=np.int32)
+        eng.check(self._lib.pte_pt_get_site(self._handle, int(index),
+                                            eng.dptr(out), count,
+                                            eng.iptr(shape)))
+        size = int(np.prod(shape))
+        return out[:size].reshape(tuple(int(v) for v in shape)).copy()
 
     def get_mpo_tensor(self, step: int) -> ndarray:
         """PT-MPO tensor ``[bond_past, bond_future, d^2]`` of ``step``
@@ -222,19 +272,24 @@ class PtTempoBackend:
         """Export every PT-MPO tensor and the caps into the process tensor
         (pt_tempo_backend.py:309-316)."""
         assert self._step >= self._num_steps
+        process_tensor = self._process_tensor_ref()
+        assert process_tensor is not None, "the process tensor was released"
         for step in reversed(range(self._num_steps)):
-            self._process_tensor.set_mpo_tensor(step,
-                                                self.get_mpo_tensor(step))
+            process_tensor.set_mpo_tensor(step, self.get_mpo_tensor(step))
         caps = self.compute_caps()
         for k, cap in enumerate(caps):
-            self._process_tensor.set_cap_tensor(k, cap)
-        self._process_tensor.attach_engine(self)
+            process_tensor.set_cap_tensor(k, cap)
+        process_tensor.attach_engine(self)
+
+    def release(self) -> None:
+        """Destroy the engine and free its device memory (idempotent)."""
+        handle = getattr(self, "_handle", None)
+        self._handle = None
+        if handle is not None and handle.value and not eng.finalizing():
+            self._lib.pte_pt_destroy(handle)
 
     def __del__(self) -> None:
-        handle = getattr(self, "_handle", None)
-        if handle is not None and handle.value:
-            try:
-                self._lib.pte_pt_destroy(handle)
-            except Exception:  # interpreter shutdown
-                pass
-            self._handle = None
+        try:
+            self.release()
+        except Exception:  # interpreter shutdown
+            pass

@@ -71,6 +71,7 @@ class TempoBackend:
         eng.check(self._lib.pte_tempo_create(ctypes.byref(cfg),
                                              ctypes.byref(handle)))
         self._handle = handle
+        eng.track(self)
         count = 1 if self._dkmax is None else self._dkmax + 1
         for dk in range(count):
             self._set_influence(dk, self._influence(dk))
@@ -122,11 +123,15 @@ class TempoBackend:
             self._handle, eng.iptr(out), int(out.size), ctypes.byref(n)))
         return [int(b) for b in out[:max(n.value - 1, 0)]]
 
-    def __del__(self) -> None:
+    def release(self) -> None:
+        """Destroy the engine and free its device memory (idempotent)."""
         handle = getattr(self, "_handle", None)
-        if handle is not None and handle.value:
-            try:
-                self._lib.pte_tempo_destroy(handle)
-            except Exception:
-                pass
-            self._handle = None
+        self._handle = None
+        if handle is not None and handle.value and not eng.finalizing():
+            self._lib.pte_tempo_destroy(handle)
+
+    def __del__(self) -> None:
+        try:
+            self.release()
+        except Exception:  # interpreter shutdown
+            pass

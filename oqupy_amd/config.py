@@ -9,8 +9,9 @@ but here every configuration selects the MI355X engine: the keys understood are
 * ``"precision"``: ``"c128"`` (complex128 everywhere, the default) or ``"c64"``
   (complex64 contractions with FP32 accumulation; FP64 for the
   orthogonalisation, singular values and the truncation decision);
-* ``"chi_cap"``: capacity of every bond in the device-resident chain
-  (default 512); a truncation that would need more raises an error.
+* ``"chi_cap"``: an optional limit on every bond of the device-resident
+  chain (default: none, as in the reference, which never caps a bond); when
+  given, a truncation that would keep more singular values raises an error.
 """
 
 from numpy import complex128, float64
@@ -41,5 +42,6 @@ PT_TEMPO_BACKEND_CONFIG = {"backend": "mi355x", "precision": "c128"}
 # differences (oqupy/config.py:75)
 DEFAULT_TOLERANCE_DEGENERACY = 12
 
-# capacity of a bond of the device-resident tensor chain
-DEFAULT_CHI_CAP = 512
+# bond limit of the device-resident tensor chain when ``chi_cap`` is not given:
+# none (the largest C int; device buffers grow with the bonds on demand)
+DEFAULT_CHI_CAP = 2**31 - 1

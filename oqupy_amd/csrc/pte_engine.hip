@@ -297,6 +297,25 @@ struct Workspace {
   int tr_m = 0, tr_n = 0, tr_r = 0;
   float tr_qr_ms = 0.f;
   double jac_ms = 0.0, jac_flops = 0.0, jac_bytes = 0.0;
+  // optional log of the truncated SVDs (pte_pt_svd_log): per SVD the shape of
+  // the matrix in the reference's orientation, the kept rank and the kept
+  // singular values (read back from `sorted` after the cut); recording stops
+  // once log_cap values are held
+  long log_cap = 0;
+  std::vector<int> log_shapes;
+  std::vector<double> log_values;
+
+  void log_svd(int m, int n, int k) {
+    if (log_cap <= 0 || (long)log_values.size() + k > log_cap) return;
+    const size_t off = log_values.size();
+    log_values.resize(off + (size_t)k);
+    HIP_CHECK(hipMemcpyAsync(log_values.data() + off, sorted, (size_t)k * sizeof(double),
+                             hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    log_shapes.push_back(m);
+    log_shapes.push_back(n);
+    log_shapes.push_back(k);
+  }
 
   void init(double eps, int cap) {
     epsrel = eps;
@@ -1337,6 +1356,7 @@ struct Chain {
         // svh[j][r] = conj(Z[r + perm[j]*Dr])
         ws->emit(ws->X.p, Dr, Dr, 1, /*dj*/ Dr, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       }
+      ws->log_svd(m, Dr, k);
       // next site: [j][q][s] = sum_r svh[j][r] next[r][q][s]
       DTensor& Bn = sites[i + 1];
       require(Bn.d[0] == Dr, PTE_ERR_ARG, "internal: bond mismatch in sweep");
@@ -1382,6 +1402,7 @@ struct Chain {
         // svh[j][l] = conj(Z[l + perm[j]*Dl])
         ws->emit(ws->X.p, Dl, Dl, 1, /*dj*/ Dl, 0, /*da*/ 1, 0, 0, 0, 1, ws->svh.p, k);
       }
+      ws->log_svd(m, Dl, k);
       // previous site: [l'][q][j] = sum_l prev[l'][q][l] svh[j][l]
       DTensor& Bp = sites[i - 1];
       require(Bp.d[2] == Dl, PTE_ERR_ARG, "internal: bond mismatch in sweep");
@@ -1472,6 +1493,7 @@ struct Chain {
         ws->emit(ws->X.p, mrows, Dl, Wl, /*dj*/ Wl, 0, /*da*/ 0, Wl, /*db*/ 1, 0, 1,
                  ws->carry2.p, k);
       }
+      ws->log_svd(ncols, mrows, k);
       std::swap(ws->carry, ws->carry2);
       Cr = Dl; Dx = k; Cv = Wl;
       set_site(ia, U);
@@ -1562,6 +1584,7 @@ struct Chain {
         ws->emit(ws->X.p, mrows, Dr, E, /*dj*/ (long)E * Dr, 0, /*da*/ 1, 0, /*db*/ Dr, 0,
                  1, ws->carry2.p, k);
       }
+      ws->log_svd(ncols, mrows, k);
       std::swap(ws->carry, ws->carry2);
       Cl = Dr; Cw = E; Dx = k;
       set_site(j, U);
@@ -1764,6 +1787,18 @@ struct PtEngine {
           out[2 * o] = v.real();
           out[2 * o + 1] = v.imag();
         }
+  }
+
+  // raw MPS site i, [left][physical][right], at any step
+  void get_site(int i, double* out, long cap, int* shape3) {
+    require(step >= 1, PTE_ERR_STATE, "engine not initialised");
+    require(i >= 0 && i < (int)mps.sites.size(), PTE_ERR_ARG, "site index out of range");
+    const DTensor& s = mps.sites[i];
+    for (int a = 0; a < 3; ++a) shape3[a] = s.d[a];
+    require(cap >= s.size(), PTE_ERR_ARG, "output buffer too small");
+    HIP_CHECK(hipMemcpyAsync(out, s.p, (size_t)s.size() * sizeof(double2),
+                             hipMemcpyDeviceToHost, st()));
+    HIP_CHECK(hipStreamSynchronize(st()));
   }
 
   // process_tensor.py:380-406, with trace_square = vec(1)/d; sites keep the
@@ -2537,6 +2572,14 @@ int pte_pt_get_mpo_tensor(pte_pt_engine* e, int step, double* out, long cap, int
   });
 }
 
+int pte_pt_get_site(pte_pt_engine* e, int i, double* out, long cap, int* shape3) {
+  return guarded([&] {
+    require(e != nullptr && out != nullptr && shape3 != nullptr, PTE_ERR_ARG,
+            "null argument");
+    e->impl.get_site(i, out, cap, shape3);
+  });
+}
+
 int pte_pt_compute_caps(pte_pt_engine* e) {
   return guarded([&] {
     require(e != nullptr, PTE_ERR_ARG, "null engine");
@@ -2567,6 +2610,35 @@ int pte_pt_last_step_stats(pte_pt_engine* e, int* n_svd, int* sweeps, int* max_b
     if (sweeps) *sweeps = e->impl.ws.sweeps;
     if (max_bond) *max_bond = e->impl.ws.max_bond;
     if (ms) *ms = e->impl.last_ms;
+  });
+}
+
+int pte_pt_svd_log(pte_pt_engine* e, long max_values) {
+  return guarded([&] {
+    require(e != nullptr && max_values >= 0, PTE_ERR_ARG, "bad engine or log size");
+    Workspace& ws = e->impl.ws;
+    ws.log_cap = max_values;
+    ws.log_shapes.clear();
+    ws.log_values.clear();
+  });
+}
+
+int pte_pt_get_svd_log(pte_pt_engine* e, int* shapes, int max_svds, double* values,
+                       long max_values, int* n_svds, long* n_values) {
+  return guarded([&] {
+    require(e != nullptr && n_svds != nullptr && n_values != nullptr, PTE_ERR_ARG,
+            "null argument");
+    const Workspace& ws = e->impl.ws;
+    const int ns = (int)(ws.log_shapes.size() / 3);
+    const long nv = (long)ws.log_values.size();
+    *n_svds = ns;
+    *n_values = nv;
+    if (shapes != nullptr)
+      std::memcpy(shapes, ws.log_shapes.data(),
+                  sizeof(int) * 3 * (size_t)std::max(0, std::min(ns, max_svds)));
+    if (values != nullptr)
+      std::memcpy(values, ws.log_values.data(),
+                  sizeof(double) * (size_t)std::max(0L, std::min(nv, max_values)));
   });
 }
 

@@ -6,6 +6,10 @@ reference has no dangling bond leg, a dimension-1 leg stands in for it: the
 matrices handed to the SVD are then identical to the reference's, because a
 length-1 axis changes no reshape.
 
+Every function that truncates takes an optional ``spectra`` list, which
+receives ``(rows, columns, singular values)`` of each SVD in order: the shape
+of the matrix the reference factors and its whole spectrum.
+
 Pinning: ``truncated_svd`` is checked against the rule restated in-tree by the
 reference (``oqupy/mps_mpo.py:452-457``) and against hand-made spectra at the
 cut boundary; zip-up and sweeps are pinned end-to-end by the golden states of
@@ -28,11 +32,14 @@ def truncation_rank(s: np.ndarray, epsrel: float) -> int:
     return int(np.count_nonzero(tail > epsrel * s[0]))
 
 
-def truncated_svd(mat: np.ndarray, epsrel: float
+def truncated_svd(mat: np.ndarray, epsrel: float, spectra: list = None
                   ) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
     """Thin SVD by LAPACK gesdd (``numpy.linalg.svd``, as tensornetwork's
-    numpy backend calls it), truncated by ``truncation_rank``."""
+    numpy backend calls it), truncated by ``truncation_rank``; ``spectra``
+    (optional) receives ``(rows, columns, all singular values)``."""
     u, s, vh = np.linalg.svd(mat, full_matrices=False)
+    if spectra is not None:
+        spectra.append((mat.shape[0], mat.shape[1], s.copy()))
     k = truncation_rank(s, epsrel)
     return u[:, :k], s[:k], vh[:k, :]
 
@@ -65,7 +72,7 @@ def identity_carry(site: np.ndarray, node: np.ndarray, right: bool
 
 
 def zip_up_left(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
-                log: list = None) -> None:
+                log: list = None, spectra: list = None) -> None:
     """Zip ``mpo`` into the last ``len(mpo)`` sites of ``mps``, right to left,
     with a truncated SVD at every site but the leftmost
     (NodeArray.zip_up, node_array.py:413-554, ``direction='left'``,
@@ -85,7 +92,8 @@ def zip_up_left(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
             assert dw == 1
             mps[ia] = t.reshape(dx, ds, dl).transpose(2, 1, 0).copy()
             break
-        u, s, vh = truncated_svd(t.reshape(dx * ds, dl * dw), epsrel)
+        u, s, vh = truncated_svd(t.reshape(dx * ds, dl * dw), epsrel,
+                                 spectra)
         if log is not None:
             log.append(len(s))
         k = len(s)
@@ -94,7 +102,7 @@ def zip_up_left(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
 
 
 def zip_up_right(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
-                 log: list = None) -> None:
+                 log: list = None, spectra: list = None) -> None:
     """Zip ``mpo`` (same length as ``mps``) into ``mps`` left to right, with a
     truncated SVD at every site but the rightmost (NodeArray.zip_up,
     ``direction='right'``, ``left_index=0``, ``right_index=-1``)."""
@@ -111,7 +119,8 @@ def zip_up_right(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
             assert dr == 1
             mps[j] = t.reshape(dx, ds, de).copy()
             break
-        u, s, vh = truncated_svd(t.reshape(dx * ds, dr * de), epsrel)
+        u, s, vh = truncated_svd(t.reshape(dx * ds, dr * de), epsrel,
+                                 spectra)
         if log is not None:
             log.append(len(s))
         k = len(s)
@@ -120,13 +129,14 @@ def zip_up_right(mps: List[np.ndarray], mpo: List[np.ndarray], epsrel: float,
 
 
 def sweep_right(mps: List[np.ndarray], start: int, stop: int, epsrel: float,
-                log: list = None) -> None:
+                log: list = None, spectra: list = None) -> None:
     """Truncated SVD sweep from site ``start`` to site ``stop`` (exclusive of
     the SVD at ``stop``), left to right (NodeArray.svd_sweep,
     node_array.py:251-273)."""
     for i in range(start, stop):
         dl, dp, dr = mps[i].shape
-        u, s, vh = truncated_svd(mps[i].reshape(dl * dp, dr), epsrel)
+        u, s, vh = truncated_svd(mps[i].reshape(dl * dp, dr), epsrel,
+                                 spectra)
         if log is not None:
             log.append(len(s))
         k = len(s)
@@ -135,14 +145,14 @@ def sweep_right(mps: List[np.ndarray], start: int, stop: int, epsrel: float,
 
 
 def sweep_left(mps: List[np.ndarray], start: int, stop: int, epsrel: float,
-               log: list = None) -> None:
+               log: list = None, spectra: list = None) -> None:
     """Truncated SVD sweep from site ``start`` down to site ``stop``
     (exclusive of the SVD at ``stop``), right to left
     (NodeArray.svd_sweep, node_array.py:274-296)."""
     for i in range(start, stop, -1):
         dl, dp, dr = mps[i].shape
         mat = mps[i].reshape(dl, dp * dr).T
-        u, s, vh = truncated_svd(mat, epsrel)
+        u, s, vh = truncated_svd(mat, epsrel, spectra)
         if log is not None:
             log.append(len(s))
         k = len(s)

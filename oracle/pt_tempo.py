@@ -51,11 +51,31 @@ class PtTempoOracle:
         self.mps: List[np.ndarray] = []
         self.mpo: List[np.ndarray] = []
         self.bond_log: List[List[int]] = []
+        # set to a list to receive (rows, columns, singular values) of every
+        # truncated SVD that follows (initialisation sweeps and steps)
+        self.spectra: Optional[list] = None
 
     # -- pt_tempo_backend.py:105-190 ------------------------------------------
     def initialize(self) -> None:
+        self.mpo, self.mps = self._build()
+        n = len(self.mps)
+        chain.sweep_left(self.mps, n - 1, 0, self.epsrel, None, self.spectra)
+        chain.sweep_right(self.mps, 0, n - 1, self.epsrel, None, self.spectra)
+        self.step = 1
+
+    def load_state(self, step: int, sites: List[np.ndarray]) -> None:
+        """Continue from the state another implementation reached after
+        ``step`` steps: the MPS sites as given (``[left, physical, right]``)
+        and the MPO as the backend holds it at that step."""
+        self.mpo, _ = self._build()
+        for s in range(2, step + 1):
+            self._advance_mpo(s)
+        self.mps = [np.array(site, dtype=complex) for site in sites]
+        self.step = step
+
+    def _build(self):
+        """The initial MPO nodes and MPS sites (before the sweeps)."""
         d = self.dim
-        self.sum_north_scaled = self.sum_north * d
         mpo, mps = [], []
         for i in range(self.num_infl):
             infl = self.influence(i)
@@ -80,32 +100,32 @@ class PtTempoOracle:
             else:
                 mpo.append(_create_delta(infl, [0, 1, 1, 0]))
                 mps.append(_create_delta(infl / d, [0, 1, 0]))
-        if self.num_infl == 1:
-            # a single node keeps its right leg open (no last-node branch)
-            pass
-        self.mpo, self.mps = mpo, mps
-        n = len(self.mps)
-        chain.sweep_left(self.mps, n - 1, 0, self.epsrel)
-        chain.sweep_right(self.mps, 0, n - 1, self.epsrel)
-        self.step = 1
+        # (a single node keeps its right leg open: no last-node branch)
+        return mpo, mps
+
+    def _advance_mpo(self, step: int) -> bool:
+        """The MPO change of ``step`` (pt_tempo_backend.py:229-257); returns
+        whether the step is in the end phase."""
+        end_phase = step > self.num_steps - self.num_infl + 1
+        if end_phase:
+            last = self.mpo[-2]
+            closed = np.einsum('wpsv,v->wps', last, self.sum_north * self.dim)
+            self.mpo = self.mpo[:-2] + [closed[..., None]]
+        else:
+            infl = self.influence(-step)
+            if infl is not None:
+                self.mpo[-1] = infl[:, None, :, None].astype(complex)
+        return end_phase
 
     # -- pt_tempo_backend.py:192-282 ------------------------------------------
     def compute_step(self) -> bool:
         self.step += 1
-        end_phase = self.step > self.num_steps - self.num_infl + 1
-        if end_phase:
-            last = self.mpo[-2]
-            closed = np.einsum('wpsv,v->wps', last, self.sum_north_scaled)
-            self.mpo = self.mpo[:-2] + [closed[..., None]]
-        else:
-            infl = self.influence(-self.step)
-            if infl is not None:
-                self.mpo[-1] = infl[:, None, :, None].astype(complex)
+        if not self._advance_mpo(self.step):
             self.mps.append(np.ones((1, 1, 1), dtype=complex))
         log: List[int] = []
-        chain.zip_up_left(self.mps, self.mpo, self.epsrel, log)
+        chain.zip_up_left(self.mps, self.mpo, self.epsrel, log, self.spectra)
         chain.sweep_right(self.mps, self.step - 2, len(self.mps) - 1,
-                          self.epsrel, log)
+                          self.epsrel, log, self.spectra)
         self.bond_log.append(log)
         return self.step < self.num_steps
 

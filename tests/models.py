@@ -142,3 +142,75 @@ def oracle_tempo(system, bath, params, rho0, num_steps, influence=None):
 def sigma_z_expectation(states):
     sz = op.sigma("z")
     return np.real(np.einsum('ij,tji->t', sz, np.asarray(states)))
+
+
+# The benchmark workloads (spin-boson / two-qubit Ohmic baths with an
+# exponential cutoff; H_S = sum_i (Omega/2) sigma_x,i [+ J sigma_z,1 sigma_z,2])
+WORKLOADS = {
+    "c1": dict(dim=2, alpha=0.3, cutoff=3.0, temp=0.0, dt=0.05, dkmax=40,
+               epsrel=1e-6, nsteps=200),
+    "c2": dict(dim=2, alpha=0.5, cutoff=5.0, temp=0.2, dt=0.05, dkmax=200,
+               epsrel=1e-7, nsteps=1000),
+    "c3": dict(dim=4, alpha=0.2, cutoff=4.0, temp=0.5, dt=0.05, dkmax=150,
+               epsrel=1e-7, nsteps=600),
+    "c5": dict(dim=2, alpha=0.6, cutoff=5.0, temp=0.1, dt=0.025, dkmax=400,
+               epsrel=1e-8, nsteps=2000),
+}
+
+
+def workload(key):
+    """(system, bath, params, spec) of a benchmark workload: one spin with
+    coupling sigma_z/2, or two spins (J = 0.5) with the collective coupling
+    (sigma_z,1 + sigma_z,2)/2 (Liouville dimension 16)."""
+    spec = WORKLOADS[key]
+    corr = PowerLawSD(alpha=spec["alpha"], zeta=1.0, cutoff=spec["cutoff"],
+                      cutoff_type="exponential", temperature=spec["temp"])
+    if spec["dim"] == 2:
+        coupling = 0.5 * op.sigma("z")
+        h_sys = 0.5 * op.sigma("x")
+    else:
+        eye = np.eye(2)
+        sz1, sz2 = np.kron(op.sigma("z"), eye), np.kron(eye, op.sigma("z"))
+        coupling = 0.5 * (sz1 + sz2)
+        h_sys = 0.5 * (np.kron(op.sigma("x"), eye)
+                       + np.kron(eye, op.sigma("x"))) + 0.5 * sz1 @ sz2
+    bath = Bath(coupling, corr)
+    system = System(h_sys)
+    params = TempoParameters(dt=spec["dt"], dkmax=spec["dkmax"],
+                             epsrel=spec["epsrel"])
+    return system, bath, params, spec
+
+
+def sweep_instance(i, seed=0):
+    """Instance i of the seeded sweep of 256 spin-boson baths: alpha ~
+    U[0.05, 0.8], w_c ~ U[1, 10], T ~ U[0, 1], dkmax ~ U{50..300} (dt 0.05,
+    epsrel 1e-7, 500 steps); returns (bath, params)."""
+    rng = np.random.default_rng(seed)
+    alphas = rng.uniform(0.05, 0.8, 256)
+    wcs = rng.uniform(1.0, 10.0, 256)
+    temps = rng.uniform(0.0, 1.0, 256)
+    dkmaxs = rng.integers(50, 301, 256)
+    corr = PowerLawSD(alpha=float(alphas[i]), zeta=1.0, cutoff=float(wcs[i]),
+                      cutoff_type="exponential", temperature=float(temps[i]))
+    bath = Bath(0.5 * op.sigma("z"), corr)
+    params = TempoParameters(dt=0.05, dkmax=int(dkmaxs[i]), epsrel=1e-7)
+    return bath, params
+
+
+def compare_svd_logs(engine_log, oracle_log, epsrel, rel_tol=1e-10):
+    """Per truncated SVD, in order: the same matrix shape, the same kept rank
+    and kept singular values within rel_tol * s_0 of the oracle's. Returns
+    the largest deviation relative to s_0."""
+    from oracle.chain import truncation_rank
+    assert len(engine_log) == len(oracle_log), \
+        (len(engine_log), len(oracle_log))
+    worst = 0.0
+    for i, ((m, n, kept), (mo, no, full)) in enumerate(zip(engine_log,
+                                                           oracle_log)):
+        assert (m, n) == (mo, no), f"SVD {i}: shape {(m, n)} vs {(mo, no)}"
+        k = truncation_rank(full, epsrel)
+        assert len(kept) == k, f"SVD {i} ({m}x{n}): kept {len(kept)} vs {k}"
+        dev = float(np.abs(kept - full[:k]).max() / full[0]) if k else 0.0
+        assert dev <= rel_tol, f"SVD {i} ({m}x{n}): |ds|/s0 = {dev:.2e}"
+        worst = max(worst, dev)
+    return worst

@@ -240,3 +240,43 @@ def test_oracle_controls_closed_form(post):
             mid = x @ u(k * dt) @ rho0 @ u(k * dt).conj().T @ x
             ref = u((j - k) * dt) @ mid @ u((j - k) * dt).conj().T
         assert np.abs(states[j] - ref).max() < 1e-12, (j, post)
+
+
+@pytest.mark.parametrize("n,dkmax,split", [(12, 6, 4), (9, 5, 7)])
+def test_load_state_continues_a_run(n, dkmax, split):
+    """The oracle continued from another run's MPS after `split` steps (the
+    benchmark's CPU leg continues from the engine's state) equals the run
+    itself, in the growth phase and across the start of the end phase."""
+    from oracle.pt_tempo import PtTempoOracle
+    _, bath, params = models.spin_boson(0.5, 5.0, 0.2, 0.05, dkmax, 1e-7)
+    infl = models.CachedInfluence(bath, params)
+    whole = PtTempoOracle(2, infl, np.ones(4), n, dkmax, 1e-7)
+    whole.compute()
+    part = PtTempoOracle(2, infl, np.ones(4), n, dkmax, 1e-7)
+    part.initialize()
+    for _ in range(split - 1):
+        part.compute_step()
+    cont = PtTempoOracle(2, infl, np.ones(4), n, dkmax, 1e-7)
+    cont.load_state(split, [s.copy() for s in part.mps])
+    while cont.compute_step():
+        pass
+    assert cont.bond_dimensions() == whole.bond_dimensions()
+    for a, b in zip(cont.mpo_tensors(), whole.mpo_tensors()):
+        assert np.array_equal(a, b)
+
+
+def test_oracle_spectra_log_matches_bond_log():
+    """The spectra log records every truncated SVD of a step in the order of
+    the bond log, with the kept counts of the truncation rule."""
+    from oracle.chain import truncation_rank
+    from oracle.pt_tempo import PtTempoOracle
+    _, bath, params = models.spin_boson(0.5, 5.0, 0.2, 0.05, 5, 1e-7)
+    infl = models.CachedInfluence(bath, params)
+    pt = PtTempoOracle(2, infl, np.ones(4), 10, 5, 1e-7)
+    pt.initialize()
+    pt.spectra = []
+    pt.compute_step()
+    pt.compute_step()
+    kept = [truncation_rank(s, 1e-7) for _, _, s in pt.spectra]
+    assert kept == pt.bond_log[-2] + pt.bond_log[-1]
+    assert all(np.all(np.diff(s) <= 0) for _, _, s in pt.spectra)
