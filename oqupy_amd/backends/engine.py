@@ -316,6 +316,7 @@ def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192):
     ``SVh = diag(S) Vh``."""
     require_gpu()
     lib = load_library()
+    entry = lib.pte_split_svd
     mat = as_c128(matrix)
     m, n = mat.shape
     r = min(m, n)
@@ -324,9 +325,9 @@ def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192):
     svh = np.empty(r * n, dtype=np.complex128)
     k = ctypes.c_int()
     sweeps = ctypes.c_int()
-    check(lib.pte_split_svd(m, n, dptr(mat), float(epsrel), int(chi_cap),
-                            dptr(u), s.ctypes.data_as(_DP), dptr(svh),
-                            ctypes.byref(k), ctypes.byref(sweeps)))
+    check(entry(m, n, dptr(mat), float(epsrel), int(chi_cap), dptr(u),
+                s.ctypes.data_as(_DP), dptr(svh), ctypes.byref(k),
+                ctypes.byref(sweeps)))
     kk = k.value
     return (u[:m * kk].reshape(m, kk), s[:kk].copy(),
             svh[:kk * n].reshape(kk, n), sweeps.value)

@@ -65,8 +65,11 @@ static int grid_for(long total, int per_block = 256) {
 struct DevBuf {
   double2* p = nullptr;
   size_t cap = 0;  // complex entries
+  bool fixed = false;  // carved from a caller's memory: never reallocated
   void ensure(size_t n, hipStream_t s) {
     if (n <= cap) return;
+    if (fixed)
+      throw Error(PTE_ERR_CAPACITY, "matrix exceeds the allocation-free workspace");
     if (p) HIP_CHECK(hipFreeAsync(p, s));
     size_t c = std::max<size_t>(n, cap * 3 / 2);
     c = std::max<size_t>(c, 256);
@@ -74,9 +77,16 @@ struct DevBuf {
     cap = c;
   }
   void release(hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p && !fixed) (void)hipFreeAsync(p, s);
     p = nullptr;
     cap = 0;
+  }
+  // point at `count` entries of fixed memory at *base, advancing *base
+  void carve(char** base, size_t count) {
+    p = reinterpret_cast<double2*>(*base);
+    cap = count;
+    fixed = true;
+    *base += (count * sizeof(double2) + 255) & ~(size_t)255;
   }
 };
 
@@ -297,6 +307,7 @@ struct Workspace {
   int tr_m = 0, tr_n = 0, tr_r = 0;
   float tr_qr_ms = 0.f;
   double jac_ms = 0.0, jac_flops = 0.0, jac_bytes = 0.0;
+  bool external = false;     // buffers carved from caller memory (init_external)
   // optional log of the truncated SVDs (pte_pt_svd_log): per SVD the shape of
   // the matrix in the reference's orientation, the kept rank and the kept
   // singular values (read back from `sorted` after the cut); recording stops
@@ -370,7 +381,18 @@ struct Workspace {
     }
   }
 
+  // Timing events are optional (null in the allocation-free workspace).
+  void mark(hipEvent_t ev) {
+    if (ev) HIP_CHECK(hipEventRecord(ev, svd_stream));
+  }
+  float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (a && b) HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+
   void destroy() {
+    if (external) return;
     if (!stream) return;
     G.release(stream); X.release(stream); V.release(stream);
     carry.release(stream); carry2.release(stream); svh.release(stream);
@@ -394,6 +416,7 @@ struct Workspace {
 
   void ensure_sv(size_t n) {
     if (n <= nsv_cap) return;
+    require(!external, PTE_ERR_CAPACITY, "matrix exceeds the allocation-free workspace");
     HIP_CHECK(hipStreamSynchronize(stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
@@ -518,19 +541,18 @@ struct Workspace {
     JacobiArgs ja = jacobi_prepare(m, n, std::min(max_coop_blocks, 1024), &grid);
     jacobi_reset();
     void* kargs[] = {&ja};
-    HIP_CHECK(hipEventRecord(evj0, svd_stream));
+    mark(evj0);
     HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
                                          dim3(grid), dim3(256), kargs, 0, svd_stream));
-    HIP_CHECK(hipEventRecord(evj1, svd_stream));
+    mark(evj1);
     jacobi_finish(m, n);
     HIP_CHECK(hipStreamSynchronize(svd_stream));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
-    return jacobi_check(m, n, ms);
+    return jacobi_check(m, n, elapsed(evj0, evj1));
   }
 
   void ensure_qr(size_t n) {
     if (n <= qr_cap) return;
+    require(!external, PTE_ERR_CAPACITY, "matrix exceeds the allocation-free workspace");
     HIP_CHECK(hipStreamSynchronize(stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
     (void)hipFree(qr_tau); (void)hipFree(qr_vscale); (void)hipFree(qr_beta);
@@ -653,18 +675,18 @@ struct Workspace {
     qa.stop_at_mark = stop_at_mark ? 1 : 0;
     HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * kBarrierWords, svd_stream));
     void* kargs[] = {&qa};
-    HIP_CHECK(hipEventRecord(evq0, svd_stream));
+    mark(evq0);
     HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(qrcp_kernel),
                                          dim3(nblk), dim3(256), kargs,
                                          (size_t)m * sizeof(double2), svd_stream));
-    HIP_CHECK(hipEventRecord(evq1, svd_stream));
+    mark(evq1);
     HIP_CHECK(hipMemcpyAsync(qr_rank_h, qr_rank_d, sizeof(int), hipMemcpyDeviceToHost,
                              svd_stream));
     if (rank)
       HIP_CHECK(hipMemcpyAsync(rank->out_h, rank->out_d, 4 * sizeof(double),
                                hipMemcpyDeviceToHost, svd_stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
-    HIP_CHECK(hipEventElapsedTime(ms, evq0, evq1));
+    *ms = elapsed(evq0, evq1);
     return *qr_rank_h;
   }
 
@@ -714,16 +736,14 @@ struct Workspace {
   int svd_small(int m, int n, bool adaptive = false) {
     SmallSvdArgs sa = small_prepare(m, n, nullptr);
     if (adaptive) sa.stop_rel = rank->theta * epsrel;
-    HIP_CHECK(hipEventRecord(evj0, svd_stream));
+    mark(evj0);
     small_svd_kernel<<<1, 256, small_lds(m, n), svd_stream>>>(sa);
-    HIP_CHECK(hipEventRecord(evj1, svd_stream));
+    mark(evj1);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
                              svd_stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, evj0, evj1));
-    return small_finish(m, n, ms);
+    return small_finish(m, n, elapsed(evj0, evj1));
   }
 
   int qr_natural_grid(int n) const { return std::max(1, std::min({n, qr_blocks, num_cus})); }
@@ -829,8 +849,12 @@ struct Workspace {
     zgemm_kernel<<<grid, 256, 0, stream>>>(g);
   }
 
-  // A network contraction of the zip-up or a sweep: complex64 on the matrix
-  // cores in the precision form, complex128 otherwise.
+  // The network contraction of a zip-up site: complex64 on the matrix cores
+  // in the precision form, complex128 otherwise. (The sweeps' absorption of
+  // S V^H into the neighbouring site is complex128 in every form: it carries
+  // the full dynamic range of the kept singular values into the state, and
+  // rounding it to complex64 at every site of every step set the form's
+  // error floor.)
   void contract(int M, int N, int K, const double2* A, long sai, long sak,
                 const double2* B, long sbk, long sbj, double2* C, long sci, long scj) {
     if (precision != PTE_PRECISION_C64) {
@@ -1362,7 +1386,8 @@ struct Chain {
       require(Bn.d[0] == Dr, PTE_ERR_ARG, "internal: bond mismatch in sweep");
       const int Pq = Bn.d[1], Ds = Bn.d[2];
       DTensor Nn = dalloc(k, Pq, Ds, 1, st());
-      ws->contract(k, Pq * Ds, Dr, ws->svh.p, Dr, 1, Bn.p, (long)Pq * Ds, 1, Nn.p,
+      // the absorption of S V^H stays complex128 in every precision form
+      ws->gemm(k, Pq * Ds, Dr, ws->svh.p, Dr, 1, Bn.p, (long)Pq * Ds, 1, Nn.p,
                (long)Pq * Ds, 1);
       set_site(i, U);
       set_site(i + 1, Nn);
@@ -1408,7 +1433,7 @@ struct Chain {
       require(Bp.d[2] == Dl, PTE_ERR_ARG, "internal: bond mismatch in sweep");
       const int Dl2 = Bp.d[0], Pq = Bp.d[1];
       DTensor Np = dalloc(Dl2, Pq, k, 1, st());
-      ws->contract(Dl2 * Pq, k, Dl, Bp.p, Dl, 1, ws->svh.p, 1, Dl, Np.p, k, 1);
+      ws->gemm(Dl2 * Pq, k, Dl, Bp.p, Dl, 1, ws->svh.p, 1, Dl, Np.p, k, 1);
       set_site(i, U);
       set_site(i - 1, Np);
     }

@@ -20,6 +20,22 @@ from typing import Callable, List, Optional, Sequence
 import numpy as np
 
 
+def pt_transforms(unitary: np.ndarray):
+    """Basis transforms of a process tensor whose bath coupling is diagonal
+    in the basis of ``unitary`` (columns = eigenvectors), as
+    ``oqupy/pt_tempo.py:157-174`` builds them: ``transform_in`` takes a
+    row-major vectorised system state rho to vec(U^H rho U) and
+    ``transform_out`` takes it back (vec(U rho U^H)), both stored transposed
+    for the leg contractions of ``expand_mpo``; None for a diagonal coupling.
+    Row-major vec(A X B) = kron(A, B^T) vec(X)."""
+    u = np.asarray(unitary, dtype=complex)
+    if np.allclose(u, np.identity(u.shape[0])):
+        return None, None
+    to_eigen = np.kron(u.conj().T, u.T)
+    from_eigen = np.kron(u, u.conj())
+    return to_eigen.T, from_eigen.T
+
+
 def expand_mpo(tensor: np.ndarray, transform_in: Optional[np.ndarray],
                transform_out: Optional[np.ndarray]) -> np.ndarray:
     """Rank-3 ``[a, b, i]`` PT tensor to the rank-4 ``[a, b, in, out]`` form

@@ -126,7 +126,7 @@ def test_pt_tempo_non_diagonal_coupling():
     rho0 = op.spin_dm("x+")
     n = 10
     pt, dyn = _engine_pt(bath, params, n * 0.1 + 1e-11, rho0, system)
-    from oracle.dynamics import compute_dynamics
+    from oracle.dynamics import compute_dynamics, pt_transforms
     from oracle.pt_tempo import PtTempoOracle, compute_caps
     infl = models.CachedInfluence(bath, params)
     ref = PtTempoOracle(2, infl, np.ones(4), n, 5, 1e-6)
@@ -134,9 +134,9 @@ def test_pt_tempo_non_diagonal_coupling():
     tens = ref.mpo_tensors()
     caps = compute_caps(tens, 2)
     props = system.get_propagators(0.1, 0.0, None, None)
+    # the oracle builds its own transforms from the bath's eigenbasis
     states = compute_dynamics(props, rho0, n, [tens], [caps],
-                              transforms=[(pt.transform_in,
-                                           pt.transform_out)])
+                              transforms=[pt_transforms(bath.unitary_transform)])
     assert np.abs(dyn.states - np.array(states)).max() < STATE_TOL
 
 
@@ -288,8 +288,10 @@ def _oracle_tempo_spread(system, bath, params, rho0, n):
     ref, _ = models.oracle_tempo(system, bath, params, rho0, n, infl)
     orig = chain.truncated_svd
 
-    def gesvd(mat, eps):
+    def gesvd(mat, eps, spectra=None):
         u, s, vh = sl.svd(mat, full_matrices=False, lapack_driver="gesvd")
+        if spectra is not None:
+            spectra.append((mat.shape[0], mat.shape[1], s.copy()))
         k = chain.truncation_rank(s, eps)
         return u[:, :k], s[:k], vh[:k]
     chain.truncated_svd = gesvd

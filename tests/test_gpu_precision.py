@@ -21,8 +21,9 @@ pytestmark = pytest.mark.gpu
 # single-precision contractions: |C - AB| <= K * 2^-24 * |A| |B| per entry;
 # the bound leaves a factor ~10 over that worst case
 C64_GEMM_REL = 1e-5
-# max |<sigma_z>_c64 - <sigma_z>_oracle| over the trajectory
-C64_SIGMA_Z_TOL = 1e-5
+# max |<sigma_z>_c64 - <sigma_z>_oracle| over the trajectory: the form's
+# stated accuracy
+C64_SIGMA_Z_TOL = 1e-6
 
 
 def _announce(name):

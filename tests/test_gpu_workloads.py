@@ -5,11 +5,11 @@
   200-step run, states within the LAPACK-driver spread rule of the TEMPO
   golden tests (1e-10 until two LAPACK drivers disagree beyond 1e-12).
 * PT-TEMPO, spin-boson alpha 0.5, w_c 5, T 0.2, dkmax 200, epsrel 1e-7
-  (N = 1000): initialisation and the first 10 steps.
+  (N = 1000): initialisation and the first 20 steps.
 * PT-TEMPO, two qubits with a collective bath (Liouville dimension 16),
-  dkmax 150, epsrel 1e-7 (N = 600): initialisation and the first 6 steps.
+  dkmax 150, epsrel 1e-7 (N = 600): initialisation and the first 8 steps.
 * PT-TEMPO, long memory alpha 0.6, dt 0.025, dkmax 400, epsrel 1e-8
-  (N = 2000): initialisation and the first 6 steps.
+  (N = 2000): initialisation and the first 12 steps.
 * The batched form on the first 32 instances of the seeded sweep of baths
   (dkmax 50..300): 3 steps of every instance through pte_pt_batch_run.
 
@@ -95,18 +95,18 @@ def _pt_workload_parity(key, steps):
 
 
 def test_c2_headline_first_steps():
-    _announce("workload c2: PT-TEMPO dkmax=200 epsrel=1e-7, init + 10 steps")
-    _pt_workload_parity("c2", 10)
+    _announce("workload c2: PT-TEMPO dkmax=200 epsrel=1e-7, init + 20 steps")
+    _pt_workload_parity("c2", 20)
 
 
 def test_c3_two_qubits_first_steps():
-    _announce("workload c3: two qubits d^2=16 dkmax=150, init + 6 steps")
-    _pt_workload_parity("c3", 6)
+    _announce("workload c3: two qubits d^2=16 dkmax=150, init + 8 steps")
+    _pt_workload_parity("c3", 8)
 
 
 def test_c5_long_memory_first_steps():
-    _announce("workload c5: dt=0.025 dkmax=400 epsrel=1e-8, init + 6 steps")
-    _pt_workload_parity("c5", 6)
+    _announce("workload c5: dt=0.025 dkmax=400 epsrel=1e-8, init + 12 steps")
+    _pt_workload_parity("c5", 12)
 
 
 def test_c1_tempo_whole_run():
