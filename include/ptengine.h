@@ -272,6 +272,20 @@ int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap,
                   double* U, double* S, double* SVh, int* k_out,
                   int* sweeps_out);
 
+/* The truncated SVD of pte_split_svd without allocating: no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, rule and results as pte_split_svd, for
+ * matrices of at most 256 x 256 (larger ones: PTE_ERR_CAPACITY). Its
+ * workspace is a 12 MiB file-scope device array (one per GPU, held until the
+ * process ends) that every call on a GPU reuses: calls are serialised by a
+ * lock inside the library, and a call that overlapped another on the same
+ * GPU's array would race. It runs on the null stream and calls no hipMalloc,
+ * hipFree or other allocating routine. */
+int pte_split_svd_noalloc(int m, int n, const double* M, double epsrel, int chi_cap,
+                          double* U, double* S, double* SVh, int* k_out,
+                          int* sweeps_out);
+
 /* One network contraction C = A B (A: m x k, B: k x n, C: m x n, row-major
  * complex128) by the kernels of the given precision (PTE_PRECISION_C64
  * rounds the operands to complex64 and accumulates in FP32 on the matrix

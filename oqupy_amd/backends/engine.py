@@ -112,6 +112,9 @@ _SIGNATURES = {
     "pte_split_svd": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _DP,
                                      ctypes.c_double, ctypes.c_int, _DP, _DP,
                                      _DP, _IP, _IP]),
+    "pte_split_svd_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _DP,
+                                             ctypes.c_double, ctypes.c_int, _DP,
+                                             _DP, _DP, _IP, _IP]),
     "pte_contract": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _DP, _DP, _DP]),
     "pte_pt_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int,
@@ -309,14 +312,16 @@ def contract(a: np.ndarray, b: np.ndarray, precision: str = "c128"):
     return c
 
 
-def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192):
+def split_svd(matrix: np.ndarray, epsrel: float, chi_cap: int = 8192,
+              allocate: bool = True):
     """Truncated SVD ``matrix = U diag(S) Vh`` by the engine's kernels, with
     the relative truncation rule of ``split_node_full_svd``
     (oqupy/backends/node_array.py:262). Returns ``(U, S, SVh, sweeps)`` with
-    ``SVh = diag(S) Vh``."""
+    ``SVh = diag(S) Vh``. ``allocate=False`` calls the allocation-free entry
+    point (matrices up to 256 x 256, workspace in a static device array)."""
     require_gpu()
     lib = load_library()
-    entry = lib.pte_split_svd
+    entry = lib.pte_split_svd if allocate else lib.pte_split_svd_noalloc
     mat = as_c128(matrix)
     m, n = mat.shape
     r = min(m, n)

@@ -391,6 +391,69 @@ struct Workspace {
     return ms;
   }
 
+  // A workspace for one truncated SVD of at most max_m x max_n (and
+  // max_n x max_m) in `bytes` of caller-provided device memory at `base`,
+  // on the null stream: no stream, event or memory is created, and a larger
+  // matrix is a capacity error. Host-side summaries go to host_info.
+  void init_external(double eps, int cap, char* base, size_t bytes, int max_m, int max_n,
+                     SvdInfo* host_info, int* host_rank) {
+    external = true;
+    epsrel = eps;
+    chi_cap = cap;
+    stream = nullptr;
+    svd_stream = nullptr;
+    char* const begin = base;
+    const size_t mn = (size_t)max_m * max_n;
+    const size_t big = std::max(max_m, max_n), small = std::min(max_m, max_n);
+    auto take = [&](size_t b) {
+      char* p = base;
+      base += (b + 255) & ~(size_t)255;
+      return p;
+    };
+    bar = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * kSyncWords));
+    flags = reinterpret_cast<int*>(take(sizeof(int) * max_sweeps));
+    fro2 = reinterpret_cast<double*>(take(sizeof(double)));
+    info_d = reinterpret_cast<SvdInfo*>(take(sizeof(SvdInfo)));
+    qr_cand = reinterpret_cast<QrcpCand*>(take(sizeof(QrcpCand) * 2 * 256));
+    qr_rank_d = reinterpret_cast<int*>(take(sizeof(int)));
+    qr_tau = reinterpret_cast<double2*>(take(sizeof(double2) * big));
+    qr_vscale = reinterpret_cast<double2*>(take(sizeof(double2) * big));
+    qr_beta = reinterpret_cast<double*>(take(sizeof(double) * big));
+    qr_piv = reinterpret_cast<int*>(take(sizeof(int) * big));
+    qr_done = reinterpret_cast<int*>(take(sizeof(int) * big));
+    qr_cap = big;
+    sigma = reinterpret_cast<double*>(take(sizeof(double) * big));
+    sorted = reinterpret_cast<double*>(take(sizeof(double) * big));
+    perm = reinterpret_cast<int*>(take(sizeof(int) * big));
+    nsv_cap = big;
+    Q.carve(&base, mn);
+    X.carve(&base, mn);
+    V.carve(&base, big * big);
+    G.carve(&base, mn);       // the input matrix, row-major
+    carry.carve(&base, mn);   // left factor
+    svh.carve(&base, mn);     // S V^H
+    const int npairs = jacobi_blocks((int)big) / 2;
+    gpart.carve(&base, (size_t)2 * npairs * 16 * 1024);
+    (void)small;
+    require((size_t)(base - begin) <= bytes, PTE_ERR_CAPACITY,
+            "internal: allocation-free workspace larger than its memory");
+    info_h = host_info;
+    qr_rank_h = host_rank;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(jacobi_kernel), 256, 0));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    max_coop_blocks = per_cu * prop.multiProcessorCount;
+    num_cus = prop.multiProcessorCount;
+    require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kSmallLdsMax));
+  }
+
   void destroy() {
     if (external) return;
     if (!stream) return;
@@ -3171,6 +3234,77 @@ int pte_split_svd(int m, int n, const double* M, double epsrel, int chi_cap, dou
       throw;
     }
     ws.destroy();
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free truncated SVD (pte_split_svd_noalloc): the workspace is this
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code never uses the array's name as an address: each device's address
+// comes from hipGetSymbolAddress on its first call.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxDim = 256;
+constexpr size_t kNoAllocBytes = (size_t)12 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_noalloc_mutex;
+constexpr int kNoAllocMaxDevices = 64;
+static char* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// device address of this GPU's scratch (caller holds g_noalloc_mutex)
+static char* noalloc_scratch() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  require(dev >= 0 && dev < kNoAllocMaxDevices, PTE_ERR_DEVICE, "device index out of range");
+  if (g_noalloc_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, PTE_ERR_DEVICE, "no address for the SVD scratch");
+    g_noalloc_ptr[dev] = static_cast<char*>(p);
+  }
+  return g_noalloc_ptr[dev];
+}
+
+extern "C" {
+
+int pte_split_svd_noalloc(int m, int n, const double* M, double epsrel, int chi_cap,
+                          double* U, double* S, double* SVh, int* k_out, int* sweeps_out) {
+  return guarded([&] {
+    require(m >= 1 && n >= 1 && M && U && S && SVh && k_out, PTE_ERR_ARG,
+            "bad arguments");
+    require(epsrel > 0.0 && chi_cap >= 1, PTE_ERR_ARG, "bad epsrel or capacity");
+    require(m <= kNoAllocMaxDim && n <= kNoAllocMaxDim, PTE_ERR_CAPACITY,
+            "pte_split_svd_noalloc takes matrices of at most " +
+                std::to_string(kNoAllocMaxDim) + " x " + std::to_string(kNoAllocMaxDim));
+    const long cnt = (long)m * n;
+    for (long i = 0; i < 2 * cnt; ++i)
+      require(std::isfinite(M[i]), PTE_ERR_NUMERIC, "non-finite value in an input tensor");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    char* base = noalloc_scratch();
+    SvdInfo info_host{};
+    int rank_host = 0;
+    Workspace ws;
+    ws.init_external(epsrel, chi_cap, base, kNoAllocBytes, kNoAllocMaxDim, kNoAllocMaxDim,
+                     &info_host, &rank_host);
+    HIP_CHECK(hipMemcpyAsync(ws.G.p, M, (size_t)cnt * sizeof(double2), hipMemcpyHostToDevice,
+                             ws.stream));
+    // column-major copy of M for the pivoted QR (m <= kQrMaxRows here)
+    transpose_kernel<<<grid_for(cnt), 256, 0, ws.stream>>>(ws.G.p, m, n, ws.Q.p);
+    const int k = ws.svd_qrcp(m, n);
+    ws.emit_left_q(1, 1, 0, 0, 1, 0, 0, ws.carry.p, k);
+    ws.emit_svh(1, n, 0, 1, 0, 0, 0, ws.svh.p, k);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(U, ws.carry.p, (size_t)m * k * sizeof(double2),
+                             hipMemcpyDeviceToHost, ws.stream));
+    HIP_CHECK(hipMemcpyAsync(SVh, ws.svh.p, (size_t)k * n * sizeof(double2),
+                             hipMemcpyDeviceToHost, ws.stream));
+    HIP_CHECK(hipMemcpyAsync(S, ws.sorted, (size_t)k * sizeof(double),
+                             hipMemcpyDeviceToHost, ws.stream));
+    HIP_CHECK(hipStreamSynchronize(ws.stream));
+    *k_out = k;
+    if (sweeps_out) *sweeps_out = info_host.sweeps;
   });
 }
 
