@@ -91,9 +91,17 @@ class TempoBackend:
         return self._step, copy(self._state)
 
     def compute_step(self) -> Tuple[int, ndarray]:
-        """One TEMPO step (tempo_backend.py:439-575, 619-626)."""
+        """One TEMPO step (tempo_backend.py:619-626)."""
         step = self._step + 1
         prop_1, prop_2 = self._propagators(step - 1)
+        return step, self.compute_system_step(step, prop_1, prop_2)
+
+    def compute_system_step(self, step: int, prop_1: ndarray,
+                            prop_2: ndarray) -> ndarray:
+        """Step ``step`` with the given half-step propagators; returns the
+        new state vector (BaseTempoBackend.compute_system_step,
+        tempo_backend.py:439-575)."""
+        assert step == self._step + 1, "steps must follow each other"
         extra = None
         rows = cols = 0
         if self._dkmax is None:
@@ -111,7 +119,7 @@ class TempoBackend:
                                            rows, cols, eng.dptr(state)))
         self._step = step
         self._state = state
-        return self._step, copy(self._state)
+        return copy(self._state)
 
     def bond_dimensions(self) -> List[int]:
         """Bond dimensions of the device-resident MPS."""
@@ -135,3 +143,77 @@ class TempoBackend:
             self.release()
         except Exception:  # interpreter shutdown
             pass
+
+
+class MeanFieldTempoBackend:
+    """Several TEMPO computations and a coherent field stepped together
+    (oqupy/backends/tempo_backend.py:629-773): one engine per system; the
+    field equation of motion and the field-dependent propagators are
+    evaluated on the host between the steps, where the reference evaluates
+    them."""
+
+    def __init__(self, initial_state_list: List[ndarray],
+                 initial_field: complex,
+                 influence_list: List[Callable[[int], ndarray]],
+                 unitary_transform_list: List[ndarray],
+                 propagators_list: List[Callable[[int, complex, complex],
+                                                 Tuple[ndarray, ndarray]]],
+                 compute_field: Callable, compute_field_derivative: Callable,
+                 sum_north_list: List[ndarray], sum_west_list: List[ndarray],
+                 dkmax: Optional[int], epsrel: float,
+                 config: Optional[Dict] = None,
+                 degeneracy_maps_list: Optional[List] = None,
+                 dim_list: Optional[List[int]] = None) -> None:
+        count = len(initial_state_list)
+        if degeneracy_maps_list is None:
+            degeneracy_maps_list = [None] * count
+        if dim_list is None:
+            dim_list = [None] * count
+        self._states = [np.asarray(s).reshape(-1) for s in initial_state_list]
+        self._field = complex(initial_field)
+        self._propagators_list = propagators_list
+        self._compute_field = compute_field
+        self._compute_field_derivative = compute_field_derivative
+        self._step = None
+        # the propagators of each system are passed per step
+        self._backends = [
+            TempoBackend(state, influence, unitary, None, north, west, dkmax,
+                         epsrel, config, maps, dim)
+            for state, influence, unitary, north, west, maps, dim in zip(
+                self._states, influence_list, unitary_transform_list,
+                sum_north_list, sum_west_list, degeneracy_maps_list,
+                dim_list)]
+
+    @property
+    def step(self) -> Optional[int]:
+        """The current step (None before ``initialize``)."""
+        return self._step
+
+    def initialize(self) -> Tuple[int, List[ndarray], complex]:
+        """Initialise every system's network (tempo_backend.py:749-754)."""
+        for backend in self._backends:
+            backend.initialize()
+        self._step = 0
+        return self._step, [copy(s) for s in self._states], self._field
+
+    def compute_step(self) -> Tuple[int, List[ndarray], complex]:
+        """One step of every system with the field and its derivative at the
+        start of the step, then the field's own step
+        (tempo_backend.py:756-773)."""
+        step = self._step
+        states = [copy(s) for s in self._states]
+        field = self._field
+        derivative = self._compute_field_derivative(step, states, field)
+        props = [prop(step, field, derivative)
+                 for prop in self._propagators_list]
+        next_states = [backend.compute_system_step(step + 1, *pair)
+                       for backend, pair in zip(self._backends, props)]
+        self._field = self._compute_field(step, states, field, next_states)
+        self._states = next_states
+        self._step = step + 1
+        return self._step, [copy(s) for s in self._states], self._field
+
+    def release(self) -> None:
+        """Destroy every system's engine."""
+        for backend in self._backends:
+            backend.release()

@@ -117,3 +117,81 @@ class Dynamics(BaseDynamics):
     def shape(self):
         """The shape of the states."""
         return np.copy(self._shape)
+
+
+class MeanFieldDynamics(BaseAPIClass):
+    """The time evolution of the systems of a mean-field system and of their
+    coherent field (oqupy/dynamics.py:186-296): one ``Dynamics`` per system
+    plus the field at every time."""
+
+    def __init__(self, times: Optional[List[float]] = None,
+                 system_states_list: Optional[List[List[ndarray]]] = None,
+                 fields: Optional[List[complex]] = None,
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        super().__init__(name, description)
+        self._times = []
+        self._fields = []
+        self._system_dynamics = []
+        times = [] if times is None else times
+        fields = [] if fields is None else fields
+        system_states_list = [] if system_states_list is None \
+            else system_states_list
+        assert isinstance(times, list), "Argument `times` must be a list."
+        assert isinstance(fields, list), "Argument `fields` must be a list."
+        assert isinstance(system_states_list, list), \
+            "Argument `system_states_list` must be a list."
+        assert len(times) == len(fields) == len(system_states_list), \
+            "Lists `times`, `system_states_list` and `fields` must have the " \
+            "same length."
+        for time, states, field in zip(times, system_states_list, fields):
+            self.add(time, states, field)
+
+    def add(self, time: float, system_states: List[ndarray],
+            field: complex) -> None:
+        """Insert every system's state and the field at ``time``."""
+        try:
+            t = float(time)
+        except Exception as err:
+            raise AssertionError("Argument `time` must be float.") from err
+        try:
+            fld = complex(field)
+        except Exception as err:
+            raise AssertionError("Argument `field` must be complex.") from err
+        if not self._system_dynamics:
+            self._system_dynamics = [Dynamics(name=f"system {i}")
+                                     for i in range(len(system_states))]
+        else:
+            assert len(system_states) == len(self._system_dynamics), (
+                f"Number of states to add ({len(system_states)}) does not "
+                "match number of Dynamics objects in MeanFieldDynamics "
+                f"({len(self._system_dynamics)})")
+        index = bisect(self._times, t)
+        self._times.insert(index, t)
+        self._fields.insert(index, fld)
+        for dyn, state in zip(self._system_dynamics, system_states):
+            dyn.add(t, state)
+
+    def __len__(self) -> int:
+        return len(self._times)
+
+    @property
+    def times(self) -> ndarray:
+        """Times of the dynamics."""
+        return np.array(self._times, dtype=NpDtypeReal)
+
+    @property
+    def fields(self) -> ndarray:
+        """The field at every time."""
+        return np.array(self._fields, dtype=NpDtype)
+
+    def field_expectations(self):
+        """``(times, fields)``, or ``(None, None)`` when empty."""
+        if len(self) == 0:
+            return None, None
+        return self.times, self.fields
+
+    @property
+    def system_dynamics(self) -> List[Dynamics]:
+        """One Dynamics per system."""
+        return self._system_dynamics

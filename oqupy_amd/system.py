@@ -205,3 +205,160 @@ class TimeDependentSystem(BaseSystem):
     def lindblad_operators(self):
         """Lindblad operator callables."""
         return copy(self._lindblad_operators)
+
+
+class TimeDependentSystemWithField(BaseSystem):
+    """A system whose Hamiltonian depends on time and on a complex coherent
+    field, with optional time-dependent Lindblad terms: one member of a
+    ``MeanFieldSystem`` (oqupy/system.py:288-473).
+
+    Within a time step the field is taken linear in time from its value and
+    derivative at the start of the step, and the half-step propagators are
+    formed from the Liouvillian with that field, sampled at the quarter
+    points (``subdiv_limit is None``) or integrated over each half step."""
+
+    def __init__(self, hamiltonian: Callable[[float, complex], ndarray],
+                 gammas: Optional[List[Callable[[float], float]]] = None,
+                 lindblad_operators: Optional[
+                     List[Callable[[float], ndarray]]] = None,
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        try:
+            dim = _check_hamiltonian(hamiltonian(1.0, 1.0 + 1.0j)).shape[0]
+        except Exception as err:
+            raise AssertionError("Time and field dependent Hamiltonian must "
+                                 "be vectorizable callable.") from err
+        self._hamiltonian = hamiltonian
+        gammas, lindblad_operators = _check_dissipator_lists(
+            gammas, lindblad_operators)
+        try:
+            for gamma in gammas:
+                float(gamma(1.0))
+        except Exception as err:
+            raise AssertionError("All elements of `gammas` must be "
+                                 "vectorizable callables returning floats.") \
+                from err
+        try:
+            for op in lindblad_operators:
+                np.array(op(1.0))
+        except Exception as err:
+            raise AssertionError("All elements of `lindblad_operators` must "
+                                 "be vectorizable callables returning numpy "
+                                 "arrays.") from err
+        self._gammas = list(gammas)
+        self._lindblad_operators = list(lindblad_operators)
+        super().__init__(dim, name, description)
+
+    def liouvillian(self, t0: float, t: float, field: complex,
+                    field_derivative: complex) -> ndarray:
+        """The Liouvillian at time ``t`` with the field extrapolated
+        linearly from its value ``field`` and derivative
+        ``field_derivative`` at ``t0 <= t``."""
+        try:
+            t0 = float(t0)
+        except Exception as err:
+            raise TypeError("Argument t0 must be float") from err
+        try:
+            t = float(t)
+        except Exception as err:
+            raise TypeError("Argument t must be float") from err
+        assert t >= t0, "Argument t must equal or exceed t0"
+        try:
+            field = complex(field)
+        except Exception as err:
+            raise TypeError("Argument field must be complex") from err
+        try:
+            field_derivative = complex(field_derivative)
+        except Exception as err:
+            raise TypeError("Argument field_derivative must be complex") \
+                from err
+        ham = np.array(self._hamiltonian(t, field + field_derivative
+                                         * (t - t0)), dtype=NpDtype)
+        gammas = [float(g(t)) for g in self._gammas]
+        ops = [np.array(op(t), dtype=NpDtype) for op in
+               self._lindblad_operators]
+        return _liouvillian(ham, gammas, ops)
+
+    def get_propagators(self, dt, start_time, subdiv_limit, epsrel):
+        """``propagators(step, field, field_derivative)`` -> the two
+        half-step propagators of ``step``."""
+        if subdiv_limit is None:
+            def propagators(step: int, field: complex,
+                            field_derivative: complex):
+                t = start_time + step * dt
+                first = self.liouvillian(t, t + dt / 4.0, field,
+                                         field_derivative)
+                second = self.liouvillian(t, t + dt * 3.0 / 4.0, field,
+                                          field_derivative)
+                return expm(first * dt / 2.0), expm(second * dt / 2.0)
+        else:
+            def propagators(step: int, field: complex,
+                            field_derivative: complex):
+                t = start_time + step * dt
+
+                def liou(tau):
+                    return self.liouvillian(t, tau, field, field_derivative)
+                first = integrate.quad_vec(liou, a=t, b=t + dt / 2.0,
+                                           epsrel=epsrel,
+                                           limit=subdiv_limit)[0]
+                second = integrate.quad_vec(liou, a=t + dt / 2.0, b=t + dt,
+                                            epsrel=epsrel,
+                                            limit=subdiv_limit)[0]
+                return expm(first), expm(second)
+        return propagators
+
+    @property
+    def hamiltonian(self) -> Callable[[float, complex], ndarray]:
+        """The Hamiltonian callable H(t, field)."""
+        return copy(self._hamiltonian)
+
+    @property
+    def gammas(self):
+        """Lindblad rate callables."""
+        return copy(self._gammas)
+
+    @property
+    def lindblad_operators(self):
+        """Lindblad operator callables."""
+        return copy(self._lindblad_operators)
+
+
+class MeanFieldSystem(BaseAPIClass):
+    """Systems coupled to one coherent field whose equation of motion
+    ``field_eom(t, [rho_1, rho_2, ...], field) -> d field / dt`` depends on
+    every system's state (oqupy/system.py:642-690)."""
+
+    def __init__(self, system_list: List[TimeDependentSystemWithField],
+                 field_eom: Callable[[float, List[ndarray], complex], complex],
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        super().__init__(name, description)
+        assert isinstance(system_list, list), "Parameter system_list must " \
+            "be a list of TimeDependentSystemWithField objects."
+        assert len(system_list) > 0, "Parameter system_list must contain at " \
+            "least one TimeDependentSystemWithField"
+        for obj in system_list:
+            assert isinstance(obj, TimeDependentSystemWithField), "Each " \
+                "element of system_list must be a " \
+                "TimeDependentSystemWithField object."
+        self._system_list = list(system_list)
+        dims = [s.dimension for s in self._system_list]
+        tests = [np.identity(d, dtype=NpDtype) / d for d in dims]
+        try:
+            complex(field_eom(1.0, tests, 1.0 + 1.0j))
+        except Exception as err:
+            raise AssertionError(
+                "Field equation of motion must take a time, a list of "
+                f"matrices with shapes {[(d, d) for d in dims]} and a "
+                "complex field, and return a complex number.") from err
+        self._field_eom = field_eom
+
+    @property
+    def system_list(self) -> List[TimeDependentSystemWithField]:
+        """The systems coupled to the field."""
+        return list(self._system_list)
+
+    @property
+    def field_eom(self) -> Callable[[float, List[ndarray], complex], complex]:
+        """The field equation of motion."""
+        return copy(self._field_eom)

@@ -1,10 +1,11 @@
 """TEMPO: parameters, influence tensors and the time-stepping frontend.
 
 Restates the parts of ``oqupy/tempo.py`` on the process-tensor path:
-``TempoParameters`` (50-216), ``Tempo`` (277-491), ``influence_matrix``
-(969-1020), ``tempo_compute`` (1142-1209) and ``_parameter_memory_input_parse``
-(1254-1276). The step itself runs in the MI355X engine
-(``oqupy_amd.backends.tempo_backend.TempoBackend``).
+``TempoParameters`` (50-216), ``Tempo`` (277-491), ``MeanFieldTempo``
+(665-960), ``influence_matrix`` (969-1020), ``tempo_compute`` (1142-1209) and
+``_parameter_memory_input_parse`` (1254-1276). The step itself runs in the
+MI355X engine (``oqupy_amd.backends.tempo_backend.TempoBackend``, one per
+system for the mean-field form).
 """
 
 from copy import copy
@@ -18,9 +19,12 @@ from oqupy_amd.bath import Bath
 from oqupy_amd.bath_correlations import BaseCorrelations
 from oqupy_amd.config import (INTEGRATE_EPSREL, SUBDIV_LIMIT,
                               TEMPO_BACKEND_CONFIG)
-from oqupy_amd.dynamics import Dynamics
-from oqupy_amd.system import BaseSystem, System, TimeDependentSystem
-from oqupy_amd.util import check_isinstance, check_true, get_progress
+from oqupy_amd.dynamics import Dynamics, MeanFieldDynamics
+from oqupy_amd.system import (BaseSystem, MeanFieldSystem, System,
+                              TimeDependentSystem,
+                              TimeDependentSystemWithField)
+from oqupy_amd.util import (check_convert, check_isinstance, check_true,
+                            get_progress)
 
 
 class TempoParameters(BaseAPIClass):
@@ -222,9 +226,14 @@ def _check_time(end_time) -> float:
         raise TypeError("End time must be a float.") from err
 
 
-def _tempo_physical_input_parse(system, initial_state, bath):
-    """oqupy/tempo.py:1278-1300 (without field systems)."""
-    check_isinstance(system, (System, TimeDependentSystem), "system")
+def _tempo_physical_input_parse(system, initial_state, bath,
+                                with_field=False):
+    """oqupy/tempo.py:1278-1300; ``with_field`` takes a system of a
+    mean-field system."""
+    if with_field:
+        check_isinstance(system, TimeDependentSystemWithField, "system")
+    else:
+        check_isinstance(system, (System, TimeDependentSystem), "system")
     hs_dim = system.dimension
     check_isinstance(initial_state, ndarray, "initial_state")
     check_true(initial_state.shape == (hs_dim, hs_dim),
@@ -357,3 +366,144 @@ def tempo_compute(system: BaseSystem, bath: Bath, initial_state: ndarray,
                   backend_config, name, description)
     tempo.compute(end_time, progress_type=progress_type)
     return tempo.get_dynamics()
+
+
+class MeanFieldTempo(BaseAPIClass):
+    """TEMPO for every system of a ``MeanFieldSystem`` together with the
+    evolution of their coherent field (oqupy/tempo.py:665-960). Each system
+    has its own bath and TEMPO network on the MI355X engine; per step the
+    field derivative at the step's start sets the field-dependent
+    propagators of every system, and the field itself advances by the
+    second-order Runge-Kutta rule over the step."""
+
+    def __init__(self, mean_field_system: MeanFieldSystem,
+                 bath_list: List[Bath], parameters: TempoParameters,
+                 initial_state_list: List[ndarray], initial_field: complex,
+                 start_time: Optional[float] = 0.0,
+                 unique: Optional[bool] = False,
+                 backend_config: Optional[Dict] = None,
+                 name: Optional[Text] = None,
+                 description: Optional[Text] = None) -> None:
+        assert isinstance(mean_field_system, MeanFieldSystem), \
+            "Argument 'mean_field_system' must be an instance of " \
+            "MeanFieldSystem."
+        self._mean_field_system = mean_field_system
+        self._dynamics = None
+        self._backend_config = TEMPO_BACKEND_CONFIG if backend_config is None \
+            else backend_config
+        assert isinstance(parameters, TempoParameters), \
+            "Argument 'parameters' must be an instance of TempoParameters."
+        self._parameters = parameters
+        assert isinstance(unique, bool), "Argument 'unique' must be a boolean."
+        self._unique = unique
+        super().__init__(name, description)
+        assert isinstance(bath_list, list), \
+            "bath_list must be a list of Bath objects"
+        assert isinstance(initial_state_list, list), \
+            "initial_state_list must be a list of state matrices"
+        systems = mean_field_system.system_list
+        assert len(systems) == len(bath_list) == len(initial_state_list), (
+            f"The lengths of bath_list ({len(bath_list)}) and "
+            f"initial_state_list ({len(initial_state_list)}) must match the "
+            f"number ({len(systems)}) of systems in mean_field_system.")
+        parsed = [_tempo_physical_input_parse(sys_, rho, bath, with_field=True)
+                  for sys_, rho, bath in zip(systems, initial_state_list,
+                                             bath_list)]
+        self._systems = [p[0] for p in parsed]
+        self._initial_states = [p[1] for p in parsed]
+        self._baths = [p[2] for p in parsed]
+        self._dims = [p[3] for p in parsed]
+        self._initial_field = check_convert(initial_field, complex,
+                                            "initial_field")
+        self._start_time = check_convert(start_time, float, "start_time")
+        self._prepare_backend()
+
+    def _influence(self, bath: Bath):
+        positions = degeneracy_positions(bath) if self._unique else None
+
+        def influence(dk: int) -> ndarray:
+            return influence_matrix(dk, parameters=self._parameters,
+                                    correlations=bath.correlations,
+                                    coupling_acomm=bath.coupling_acomm,
+                                    coupling_comm=bath.coupling_comm,
+                                    deg_positions=positions)
+        return influence
+
+    def _prepare_backend(self) -> None:
+        """oqupy/tempo.py:770-830, with one engine per system."""
+        from oqupy_amd.backends.tempo_backend import MeanFieldTempoBackend
+        par = self._parameters
+        props = [s.get_propagators(par.dt, self._start_time,
+                                   par.subdiv_limit, par.liouvillian_epsrel)
+                 for s in self._systems]
+        if self._unique:
+            norths = [np.ones(np.max(b.north_degeneracy_map) + 1)
+                      for b in self._baths]
+            wests = [np.ones(np.max(b.west_degeneracy_map) + 1)
+                     for b in self._baths]
+            maps = [[b.north_degeneracy_map, b.west_degeneracy_map]
+                    for b in self._baths]
+        else:
+            norths = [np.ones(d ** 2) for d in self._dims]
+            wests = [np.ones(d ** 2) for d in self._dims]
+            maps = [None] * len(self._baths)
+        self._backend_instance = MeanFieldTempoBackend(
+            [rho.reshape(-1) for rho in self._initial_states],
+            self._initial_field, [self._influence(b) for b in self._baths],
+            [b.unitary_transform for b in self._baths], props,
+            self._compute_field, self._compute_field_derivative, norths,
+            wests, par.dkmax, par.epsrel, config=self._backend_config,
+            degeneracy_maps_list=maps, dim_list=self._dims)
+
+    def _time(self, step: int) -> float:
+        return self._start_time + float(step) * self._parameters.dt
+
+    def _matrices(self, states: List[ndarray]) -> List[ndarray]:
+        return [np.asarray(s).reshape(d, d) for s, d in zip(states, self._dims)]
+
+    def _compute_field(self, step: int, state_list: List[ndarray],
+                       field: complex, next_state_list: List[ndarray]
+                       ) -> complex:
+        """The field at step + 1 by second-order Runge-Kutta (Heun) with the
+        states at both ends of the step (oqupy/tempo.py:867-882)."""
+        dt = self._parameters.dt
+        t = self._time(step)
+        eom = self._mean_field_system.field_eom
+        slope_0 = eom(t, self._matrices(state_list), field)
+        slope_1 = eom(t + dt, self._matrices(next_state_list),
+                      field + slope_0 * dt)
+        return field + dt * (slope_0 + slope_1) / 2
+
+    def _compute_field_derivative(self, step: int,
+                                  state_list: List[ndarray],
+                                  field: complex) -> complex:
+        """d field / dt at ``step`` (oqupy/tempo.py:884-892)."""
+        return self._mean_field_system.field_eom(
+            self._time(step), self._matrices(state_list), field)
+
+    def compute(self, end_time: float, progress_type: Text = None
+                ) -> MeanFieldDynamics:
+        """Propagate (or continue) every system and the field up to
+        ``end_time`` (oqupy/tempo.py:894-940)."""
+        end = _check_time(end_time)
+        if self._backend_instance.step is None:
+            step, states, field = self._backend_instance.initialize()
+            self._dynamics = MeanFieldDynamics(
+                description=f"computed from '{self.name}' MeanFieldTempo")
+            self._dynamics.add(self._time(step), self._matrices(states), field)
+        start = self._backend_instance.step
+        num_step = max(0, int((end - self._start_time) / self._parameters.dt)
+                       - start)
+        with get_progress(progress_type)(
+                num_step, "--> TEMPO-with-field computation:") as prog:
+            for i in range(num_step):
+                prog.update(i)
+                step, states, field = self._backend_instance.compute_step()
+                self._dynamics.add(self._time(step), self._matrices(states),
+                                   field)
+            prog.update(num_step)
+        return self._dynamics
+
+    def get_dynamics(self) -> MeanFieldDynamics:
+        """The mean-field dynamics computed so far."""
+        return self._dynamics
