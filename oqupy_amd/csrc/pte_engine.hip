@@ -276,6 +276,9 @@ struct Workspace {
   int* perm = nullptr;
   size_t nsv_cap = 0;
   unsigned* bar = nullptr;
+  unsigned* jskip = nullptr;   // converged-pair bookkeeping of the Jacobi
+  size_t jskip_cap = 0;        // words
+  int jskip_words = 0;         // words the last prepared Jacobi uses
   int* flags = nullptr;
   double* fro2 = nullptr;
   SvdInfo* info_d = nullptr;
@@ -411,6 +414,11 @@ struct Workspace {
       return p;
     };
     bar = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * kSyncWords));
+    {
+      const size_t nbmax = (size_t)jacobi_blocks(std::max(max_m, max_n));
+      jskip_cap = nbmax + nbmax * (nbmax - 1) / 2;
+      jskip = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * jskip_cap));
+    }
     flags = reinterpret_cast<int*>(take(sizeof(int) * max_sweeps));
     fro2 = reinterpret_cast<double*>(take(sizeof(double)));
     info_d = reinterpret_cast<SvdInfo*>(take(sizeof(SvdInfo)));
@@ -465,7 +473,8 @@ struct Workspace {
     (void)hipFree(qr_piv); (void)hipFree(qr_done); (void)hipFree(qr_cand);
     (void)hipFree(qr_rank_d); (void)hipHostFree(qr_rank_h);
     (void)hipFree(sigma); (void)hipFree(sorted); (void)hipFree(perm);
-    (void)hipFree(bar); (void)hipFree(flags); (void)hipFree(fro2); (void)hipFree(info_d);
+    (void)hipFree(bar); (void)hipFree(jskip); (void)hipFree(flags); (void)hipFree(fro2);
+    (void)hipFree(info_d);
     (void)hipHostFree(info_h);
     (void)hipEventDestroy(ev0); (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(evj0); (void)hipEventDestroy(evj1);
@@ -513,6 +522,8 @@ struct Workspace {
     const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
     if (R > 1) gpart.ensure((size_t)2 * npairs * R * 1024, svd_stream);
     jacobi_sync_words = kBarrierWords + 2 * nb;
+    jskip_words = nb + nb * (nb - 1) / 2;
+    ensure_jskip((size_t)jskip_words);
     JacobiArgs ja;
     ja.X = X.p; ja.m = m; ja.n = n; ja.ldx = m;
     ja.V = V.p; ja.ldv = n;
@@ -525,8 +536,20 @@ struct Workspace {
     ja.inner_sweeps = inner_sweeps;
     ja.bar = bar; ja.flags = flags; ja.fro2 = fro2;
     ja.info = &info_d->sweeps;
+    ja.skip = jskip;
     *grid_out = npairs * R;
     return ja;
+  }
+
+  void ensure_jskip(size_t words) {
+    if (words <= jskip_cap) return;
+    require(!external, PTE_ERR_CAPACITY, "matrix exceeds the allocation-free workspace");
+    HIP_CHECK(hipStreamSynchronize(stream));
+    HIP_CHECK(hipStreamSynchronize(svd_stream));
+    (void)hipFree(jskip);
+    const size_t c = std::max<size_t>(words, 4096);
+    HIP_CHECK(hipMalloc((void**)&jskip, c * sizeof(unsigned)));
+    jskip_cap = c;
   }
 
   static int jacobi_blocks(int n) {
@@ -545,6 +568,7 @@ struct Workspace {
 
   void jacobi_reset() {
     HIP_CHECK(hipMemsetAsync(bar, 0, sizeof(unsigned) * jacobi_sync_words, svd_stream));
+    HIP_CHECK(hipMemsetAsync(jskip, 0, sizeof(unsigned) * jskip_words, svd_stream));
     HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * max_sweeps, svd_stream));
     HIP_CHECK(hipMemsetAsync(fro2, 0, sizeof(double), svd_stream));
   }
@@ -909,7 +933,7 @@ struct Workspace {
     g.alpha = make_double2(1.0, 0.0);
     g.beta_zero = 1;
     dim3 grid((N + 63) / 64, (M + 63) / 64);
-    zgemm_kernel<<<grid, 256, 0, stream>>>(g);
+    zgemm_mfma_kernel<<<grid, 256, 0, stream>>>(g);
   }
 
   // The network contraction of a zip-up site: complex64 on the matrix cores
@@ -1315,6 +1339,7 @@ struct Batcher {
           post.push_back(r->ws->post_args(r->jm, r->jn));
           Workspace* w = r->ws;
           zeros.push_back(ZeroItem{w->bar, w->jacobi_sync_words, 0});
+          zeros.push_back(ZeroItem{w->jskip, w->jskip_words, 0});
           zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->flags), w->max_sweeps, 0});
           zeros.push_back(ZeroItem{reinterpret_cast<unsigned*>(w->fro2), 2, 0});
           grids.push_back(g);

@@ -53,6 +53,11 @@ struct JacobiArgs {
   int* flags;
   double* fro2;
   int* info;
+  // skipping of converged pairs (zeroed before the launch): skip[b], b < nb,
+  // is 1 + the round in which block b was last rotated (0: never);
+  // skip[nb + pair_index(bi, bj)] is 1 + the round in which that block pair
+  // was last found to need no rotation (0: not since its last rotation)
+  unsigned* skip;
 };
 
 struct EmitArgs {
@@ -158,6 +163,7 @@ __global__ void small_svd_batch_kernel(const SmallSvdArgs* probs);
 __global__ void gather_qr_rank_kernel(const QrcpArgs* probs, int n, int* out);
 __global__ void apply_q_kernel(ApplyQArgs a);
 __global__ void zgemm_kernel(GemmArgs a);
+__global__ void zgemm_mfma_kernel(GemmArgs a);
 __global__ void cgemm_mfma_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);
 __global__ void mpo_apply_right_kernel(MpoApplyArgs a);

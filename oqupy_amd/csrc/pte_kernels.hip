@@ -113,6 +113,111 @@ zgemm_kernel(GemmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// complex128 GEMM on the FP64 matrix cores (the plain form's contraction):
+// C(i,j) = alpha * sum_k opA(A)(i,k) opB(B)(k,j) [+ C(i,j)], with a 3M split
+// (three real products per complex product, Gauss):
+//   P1 = Ar Br,  P2 = Ai Bi,  P3 = (Ar + Ai)(Br + Bi),
+//   Re = P1 - P2,  Im = P3 - P1 - P2,
+// accumulated in FP64 by v_mfma_f64_16x16x4_f64. 64x64 output tile per
+// 256-thread workgroup, each wavefront a 32x32 quarter as 2x2 tiles of
+// 16x16; K in slices of 16 staged through LDS as separate real and
+// imaginary planes. Operand layout of the FP64 16x16x4 instruction (measured
+// on gfx950): lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[4*r + l/16][l%16] in its item r = 0..3 (rows interleaved by 4, unlike
+// the FP32 16x16x4 instruction's D[4*(l/16)+r][l%16]).
+// ---------------------------------------------------------------------------
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+zgemm_mfma_kernel(GemmArgs a) {
+  __shared__ double Ar[GB_K][GB_M + 1], Ai[GB_K][GB_M + 1];
+  __shared__ double Br[GB_K][GB_N + 1], Bi[GB_K][GB_N + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int i0 = blockIdx.y * GB_M, j0 = blockIdx.x * GB_N;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  doublex4 p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      p1[u][v] = doublex4{0.0, 0.0, 0.0, 0.0};
+      p2[u][v] = doublex4{0.0, 0.0, 0.0, 0.0};
+      p3[u][v] = doublex4{0.0, 0.0, 0.0, 0.0};
+    }
+  const bool a_kfast = (a.sak == 1);
+  const bool b_jfast = (a.sbj == 1);
+  for (int k0 = 0; k0 < a.K; k0 += GB_K) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int row, kk;
+      if (a_kfast) { row = e >> 4; kk = e & 15; } else { row = e & 63; kk = e >> 6; }
+      const int gi = i0 + row, gk = k0 + kk;
+      double2 va = make_double2(0.0, 0.0);
+      if (gi < a.M && gk < a.K) {
+        va = a.A[(long)gi * a.sai + (long)gk * a.sak];
+        if (a.conjA) va.y = -va.y;
+      }
+      Ar[kk][row] = va.x;
+      Ai[kk][row] = va.y;
+      int col;
+      if (b_jfast) { kk = e >> 6; col = e & 63; } else { kk = e & 15; col = e >> 4; }
+      const int gj = j0 + col, gk2 = k0 + kk;
+      double2 vb = make_double2(0.0, 0.0);
+      if (gj < a.N && gk2 < a.K) {
+        vb = a.B[(long)gk2 * a.sbk + (long)gj * a.sbj];
+        if (a.conjB) vb.y = -vb.y;
+      }
+      Br[kk][col] = vb.x;
+      Bi[kk][col] = vb.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < GB_K; k4 += 4) {
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        ar[u] = Ar[k4 + lk][wr + 16 * u + lr];
+        ai[u] = Ai[k4 + lk][wr + 16 * u + lr];
+      }
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        br[v] = Br[k4 + lk][wc + 16 * v + lr];
+        bi[v] = Bi[k4 + lk][wc + 16 * v + lr];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          p1[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[u], br[v], p1[u][v], 0, 0, 0);
+          p2[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[u], bi[v], p2[u][v], 0, 0, 0);
+          p3[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[u] + ai[u], br[v] + bi[v],
+                                                          p3[u][v], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * u + 4 * r + lk;
+        const int gj = j0 + wc + 16 * v + lr;
+        if (gi < a.M && gj < a.N) {
+          const double re = p1[u][v][r] - p2[u][v][r];
+          const double im = p3[u][v][r] - p1[u][v][r] - p2[u][v][r];
+          double2 c = cmul(a.alpha, make_double2(re, im));
+          double2* dst = a.C + (long)gi * a.sci + (long)gj * a.scj;
+          if (!a.beta_zero) c = cadd(c, *dst);
+          *dst = c;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // complex GEMM in complex64 with FP32 accumulation on the matrix cores
 // (precision form): C(i,j) = sum_k A(i,k) B(k,j), inputs and output stored
 // as complex128 and rounded to complex64 when staged. 64x64 output tile per
@@ -409,6 +514,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
   __shared__ double2 Ws[JP][JP + 1];
   __shared__ int cols[JP];
   __shared__ int need;
+  __shared__ bool skip_pair;
   __shared__ double red[JT];
 
   const int tid = threadIdx.x;
@@ -462,11 +568,24 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         cols[tid] = (c < a.n) ? c : -1;
       }
       // both blocks released by the pairs that held them in the previous round
+      const int plo = min(bi, bj), phi = max(bi, bj);
+      unsigned* const pair_ok = a.skip + a.nb + ((long)phi * (phi - 1)) / 2 + plo;
       if (tid == 0) {
         while (ld_agent(blockver + bi) < gen || ld_agent(blockver + bj) < gen)
           __builtin_amdgcn_s_sleep(1);
+        // a pair found converged whose blocks have not been rotated since has
+        // the same Gram matrix: it needs no rotation and no pass over X
+        const unsigned ok = ld_agent(pair_ok);
+        skip_pair = ok != 0u && ld_agent(a.skip + bi) < ok && ld_agent(a.skip + bj) < ok;
       }
       __syncthreads();
+      if (skip_pair) {
+        // the pair barrier is kept in a skipped round: without it a slice
+        // could finish the next round before a slower slice of its pair
+        // finished this one, and the arrival count on pair_done, which
+        // releases the blocks, would then never reach this round's total
+        if (R > 1) grid_barrier(pair_bar + pair, R, pair_epoch);
+      } else {
       // ---- partial Gram of the pair's columns over this slice's rows ----
       double2 g[2][2];
 #pragma unroll
@@ -500,7 +619,9 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
 #pragma unroll
           for (int v = 0; v < 2; ++v) Gs[0][ti + 16 * u][tj + 16 * v] = g[u][v];
       } else {
-        double2* gbuf = a.gpart + (long)(gen & 1) * npairs * R * (JP * JP);
+        // double-buffered by round parity (every round of a pair, skipped or
+        // not, passes its pair barrier)
+        double2* gbuf = a.gpart + (long)(pair_epoch & 1) * npairs * R * (JP * JP);
         double2* mine = gbuf + ((long)pair * R + slice) * (JP * JP);
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -554,6 +675,17 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
       }
       __syncthreads();
       PHASE(2);
+      // record the outcome for later rounds: a rotated pair's blocks change,
+      // an unrotated pair is converged as long as they do not
+      if (slice == 0 && tid == 0) {
+        if (need) {
+          st_agent(a.skip + bi, gen + 1);
+          st_agent(a.skip + bj, gen + 1);
+          st_agent(pair_ok, 0u);
+        } else {
+          st_agent(pair_ok, gen + 1);
+        }
+      }
       if (need) {
         // ---- cyclic two-sided Jacobi on Gs, accumulating Ws ----
         // Gs is double-buffered: a sub-step reads Gs[cur] and writes every
@@ -673,6 +805,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         if (tid == 0) st_agent(a.flags + sweep, 1);
         PHASE(4);
       }
+      }   // skip_pair
       // the last slice of the pair to finish releases both blocks
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
@@ -946,6 +1079,12 @@ __device__ __forceinline__ void qrcp_post(QrcpCand* dst, double wmax, int wcol, 
   }
 }
 
+// loads of the pivoted QR issued together (per thread), and the rows of a
+// column (per lane, times 64) kept in registers between its dot product and
+// its update
+constexpr int kQrBatch = 8;
+constexpr int kQrCache = 16;
+
 __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, const int nblk) {
   extern __shared__ double2 vcol[];            // reflector of the step (m)
   __shared__ double slot[2][4], slotv[2][4];
@@ -976,7 +1115,16 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
     }
     const double2* col = a.A + (long)c * lda;
     double part = 0.0;
-    for (int i = j0 + lane; i < m; i += 64) part += cabs2(ld_agent(col + i));
+    for (int base = j0 + lane; base < m; base += 64 * kQrBatch) {
+      double2 t[kQrBatch];
+#pragma unroll
+      for (int u = 0; u < kQrBatch; ++u) {
+        const int i = base + 64 * u;
+        t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < kQrBatch; ++u) part += cabs2(t[u]);
+    }
     const double nrm = sqrt(wave_sum(part));
     if (lane == 0) {
       vn1[q] = nrm; vn2[q] = nrm; done[q] = 0;
@@ -1006,12 +1154,29 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
   }
   PHASE_INIT();
   while (j < jmax && p >= 0 && total > drop2) {
-    // reflector j from the pivot column p (rows j..m-1), whole workgroup
+    // reflector j from the pivot column p (rows j..m-1), whole workgroup: the
+    // column is read once into vcol (loads issued in batches, so that their
+    // latencies overlap), then scaled in LDS
     const double2* pc = a.A + (long)p * lda;
     double part = 0.0;
-    for (int i = j + 1 + tid; i < m; i += 256) part += cabs2(ld_agent(pc + i));
+    for (int base = j + tid; base < m; base += 256 * kQrBatch) {
+      double2 t[kQrBatch];
+#pragma unroll
+      for (int q = 0; q < kQrBatch; ++q) {
+        const int i = base + 256 * q;
+        t[q] = (i < m) ? ld_agent(pc + i) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < kQrBatch; ++q) {
+        const int i = base + 256 * q;
+        if (i < m) {
+          vcol[i] = t[q];
+          if (i > j) part += cabs2(t[q]);
+        }
+      }
+    }
     const double xnorm2 = block_sum256(part, slot, parity);
-    const double2 alpha = ld_agent(pc + j);
+    const double2 alpha = vcol[j];
     double2 tau, scale;
     double beta;
     if (xnorm2 == 0.0 && alpha.y == 0.0) {
@@ -1025,8 +1190,9 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       scale = make_double2(dr / den, -di / den);
       tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
     }
+    __syncthreads();   // every thread has read alpha = vcol[j]
     for (int i = j + tid; i < m; i += 256)
-      vcol[i] = (i == j) ? make_double2(1.0, 0.0) : cmul(ld_agent(pc + i), scale);
+      vcol[i] = (i == j) ? make_double2(1.0, 0.0) : cmul(vcol[i], scale);
     if (blk == 0 && tid == 0) {
       a.piv[j] = p;
       a.tau[j] = tau;
@@ -1047,16 +1213,61 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       if (done[q]) continue;
       const int c = blk + q * nblk;
       double2* col = a.A + (long)c * lda;
+      // rows j .. j + 64 kQrCache - 1 stay in registers between the dot
+      // product and the update; all loads are issued in batches
+      double2 cache[kQrCache];
+#pragma unroll
+      for (int u = 0; u < kQrCache; ++u) {
+        const int i = j + lane + 64 * u;
+        cache[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+      }
       if (reflect) {
         double2 w = make_double2(0.0, 0.0);
-        for (int i = j + lane; i < m; i += 64) cfmac(w, vcol[i], ld_agent(col + i));
+#pragma unroll
+        for (int u = 0; u < kQrCache; ++u) {
+          const int i = j + lane + 64 * u;
+          if (i < m) cfmac(w, vcol[i], cache[u]);
+        }
+        for (int base = j + lane + 64 * kQrCache; base < m; base += 64 * kQrBatch) {
+          double2 t[kQrBatch];
+#pragma unroll
+          for (int u = 0; u < kQrBatch; ++u) {
+            const int i = base + 64 * u;
+            t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < kQrBatch; ++u) {
+            const int i = base + 64 * u;
+            if (i < m) cfmac(w, vcol[i], t[u]);
+          }
+        }
         w.x = wave_sum(w.x);
         w.y = wave_sum(w.y);
         const double2 tw = cmul(tc, w);
-        for (int i = j + lane; i < m; i += 64)
-          st_agent(col + i, csub(ld_agent(col + i), cmul(vcol[i], tw)));
+#pragma unroll
+        for (int u = 0; u < kQrCache; ++u) {
+          const int i = j + lane + 64 * u;
+          if (i < m) {
+            cache[u] = csub(cache[u], cmul(vcol[i], tw));
+            st_agent(col + i, cache[u]);
+          }
+        }
+        for (int base = j + lane + 64 * kQrCache; base < m; base += 64 * kQrBatch) {
+          double2 t[kQrBatch];
+#pragma unroll
+          for (int u = 0; u < kQrBatch; ++u) {
+            const int i = base + 64 * u;
+            t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int u = 0; u < kQrBatch; ++u) {
+            const int i = base + 64 * u;
+            if (i < m) st_agent(col + i, csub(t[u], cmul(vcol[i], tw)));
+          }
+        }
       }
-      double2 rj2 = (lane == 0) ? ld_agent(col + j) : make_double2(0.0, 0.0);
+      // row j of the updated column sits in lane 0's first cached entry
+      double2 rj2 = cache[0];
       rj2.x = __shfl(rj2.x, 0, 64);
       rj2.y = __shfl(rj2.y, 0, 64);
       double nv = vn1[q];
@@ -1067,11 +1278,17 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
         const double ratio = nv / vn2[q];
         const double temp2 = temp * ratio * ratio;
         if (temp2 <= tol3z) {
-          // the column's new entries were stored by other lanes of this
-          // wavefront: complete the stores before reading them back
+          // recompute the norm of rows j+1.. from the cached entries and,
+          // beyond them, from memory (this lane's own stores: complete them
+          // before reading back)
           __threadfence_block();
           double pp = 0.0;
-          for (int i = j + 1 + lane; i < m; i += 64) pp += cabs2(ld_agent(col + i));
+#pragma unroll
+          for (int u = 0; u < kQrCache; ++u) {
+            const int i = j + lane + 64 * u;
+            if (i > j && i < m) pp += cabs2(cache[u]);
+          }
+          for (int i = j + lane + 64 * kQrCache; i < m; i += 64) pp += cabs2(ld_agent(col + i));
           nv = (j + 1 < m) ? sqrt(wave_sum(pp)) : 0.0;
           if (lane == 0) { vn1[q] = nv; vn2[q] = nv; }
         } else {

@@ -197,18 +197,88 @@ def sweep_instance(i, seed=0):
     return bath, params
 
 
-def compare_svd_logs(engine_log, oracle_log, epsrel, rel_tol=1e-10):
+def admissible_rank(s, k, epsrel, rel_tol):
+    """Whether the cut keeps k values for some spectrum within rel_tol * s_0
+    of s (each value moved by at most that much). The cut keeps
+    #{j: ||s[j:]|| > epsrel s_0}; moving every value by d moves ||s[j:]|| by
+    at most sqrt(n - j) d and s_0 by d, so a rank other than the rule's is
+    reached only when the tail norm at the cut lies that close to the
+    threshold: a rounding-ambiguous cut."""
+    s = np.asarray(s, dtype=float)
+    n = len(s)
+    d = rel_tol * s[0]
+    tail = np.sqrt(np.cumsum((s ** 2)[::-1])[::-1])
+
+    def tail_at(j):
+        return tail[j] if j < n else 0.0
+    if k < 0 or k > n:
+        return False
+    keeps_k_minus_1 = k == 0 or (tail_at(k - 1) + np.sqrt(n - k + 1) * d
+                                 > epsrel * (s[0] - d))
+    drops_k = tail_at(k) - np.sqrt(max(n - k, 0)) * d <= epsrel * (s[0] + d)
+    return bool(keeps_k_minus_1 and drops_k)
+
+
+class SteeredCut:
+    """While active, the oracle's truncated SVD keeps the engine's rank
+    (from the engine's SVD log, in order) wherever the oracle's own rule
+    gives another rank and the cut is rounding-ambiguous
+    (``admissible_rank``), so that both runs continue with the same bond
+    dimensions; every other SVD keeps the oracle's own rank. ``adopted``
+    lists the SVDs where the engine's rank was taken."""
+
+    def __init__(self, engine_log, epsrel, rel_tol):
+        self.engine_log = engine_log
+        self.epsrel = epsrel
+        self.rel_tol = rel_tol
+        self.adopted = []
+        self.index = 0
+
+    def __enter__(self):
+        from oracle import chain
+        self._chain = chain
+        self._orig = chain.truncated_svd
+
+        def steered(mat, eps, spectra=None):
+            u, s, vh = np.linalg.svd(mat, full_matrices=False)
+            if spectra is not None:
+                spectra.append((mat.shape[0], mat.shape[1], s.copy()))
+            k = chain.truncation_rank(s, eps)
+            i = self.index
+            self.index += 1
+            if i < len(self.engine_log):
+                k_e = len(self.engine_log[i][2])
+                if k_e != k and admissible_rank(s, k_e, eps, self.rel_tol):
+                    self.adopted.append(i)
+                    k = k_e
+            return u[:, :k], s[:k], vh[:k, :]
+        chain.truncated_svd = steered
+        return self
+
+    def __exit__(self, *exc):
+        self._chain.truncated_svd = self._orig
+        return False
+
+
+def compare_svd_logs(engine_log, oracle_log, epsrel, rel_tol=1e-10,
+                     adopted=()):
     """Per truncated SVD, in order: the same matrix shape, the same kept rank
-    and kept singular values within rel_tol * s_0 of the oracle's. Returns
-    the largest deviation relative to s_0."""
+    (or, at the SVDs listed in ``adopted``, a rank the cut reaches within
+    rel_tol * s_0 of the oracle's spectrum) and kept singular values within
+    rel_tol * s_0 of the oracle's. Returns the largest deviation relative to
+    s_0."""
     from oracle.chain import truncation_rank
     assert len(engine_log) == len(oracle_log), \
         (len(engine_log), len(oracle_log))
+    adopted = set(adopted)
     worst = 0.0
     for i, ((m, n, kept), (mo, no, full)) in enumerate(zip(engine_log,
                                                            oracle_log)):
         assert (m, n) == (mo, no), f"SVD {i}: shape {(m, n)} vs {(mo, no)}"
         k = truncation_rank(full, epsrel)
+        if i in adopted:
+            assert admissible_rank(full, len(kept), epsrel, rel_tol), i
+            k = len(kept)
         assert len(kept) == k, f"SVD {i} ({m}x{n}): kept {len(kept)} vs {k}"
         dev = float(np.abs(kept - full[:k]).max() / full[0]) if k else 0.0
         assert dev <= rel_tol, f"SVD {i} ({m}x{n}): |ds|/s0 = {dev:.2e}"

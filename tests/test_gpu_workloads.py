@@ -14,9 +14,11 @@
   (dkmax 50..300): 3 steps of every instance through pte_pt_batch_run.
 
 For PT-TEMPO every truncated SVD is compared in order (the engine's SVD log
-against the oracle's): the same matrix shape, the same kept rank, and kept
-singular values within 1e-10 s_0; the bond dimensions afterwards are
-identical.
+against the oracle's, the oracle continuing from the engine's MPS at every
+step's start): the same matrix shape, the same kept rank (a different rank
+only where the cut is rounding-ambiguous: reached by the spectrum moved by
+1e-10 s_0, and then taken by both), and kept singular values within
+1e-10 s_0; the bond dimensions afterwards are identical.
 """
 
 import ctypes
@@ -49,48 +51,67 @@ def _table(bath, params, count):
     return {dk: infl(dk) for dk in range(count)}
 
 
-def _engine_steps(bath, params, nsteps_total, table, steps, dim):
-    """Engine backend initialised and stepped `steps` times, with every SVD
-    logged."""
-    ptt = oqupy.PtTempo(bath, 0.0, nsteps_total * params.dt + 1e-9 * params.dt,
-                        params, backend_config={"svd_log": LOG_VALUES})
-    assert ptt.num_steps == nsteps_total
-    backend = ptt.backend
-    backend._influence = lambda dk: table.get(dk)
-    backend.initialize()
-    for _ in range(steps):
-        backend.compute_step()
-    return ptt, backend
-
-
-def _oracle_steps(dim, table, nsteps_total, dkmax, epsrel, steps):
-    from oracle.pt_tempo import PtTempoOracle
-    pt = PtTempoOracle(dim, lambda dk: table.get(dk), np.ones(dim * dim),
-                       nsteps_total, dkmax, epsrel)
-    pt.spectra = []
-    pt.initialize()
-    for _ in range(steps):
-        pt.compute_step()
-    return pt
+def _export(backend):
+    """The engine's MPS as stored (for the oracle to continue from)."""
+    n_sites = len(backend.bond_dimensions()) + 1
+    return backend.step, [backend.get_site(i) for i in range(n_sites)]
 
 
 def _pt_workload_parity(key, steps):
+    """Engine and oracle side by side: both initialise from the same
+    influence tables, then for every step the oracle continues from the
+    engine's MPS at the step's start (so a step's SVDs are compared on the
+    same input, not after the rounding of all earlier steps has grown
+    through the truncations), and every SVD of the whole run is compared in
+    order. Where the oracle's cut differs from the engine's only because it
+    is rounding-ambiguous, the oracle keeps the engine's rank
+    (``models.SteeredCut``)."""
+    from oracle.pt_tempo import PtTempoOracle
     system, bath, params, spec = models.workload(key)
-    dim, n, dkmax = spec["dim"], spec["nsteps"], spec["dkmax"]
+    dim, n, dkmax, epsrel = (spec["dim"], spec["nsteps"], spec["dkmax"],
+                             spec["epsrel"])
     t0 = time.perf_counter()
     table = _table(bath, params, min(n, dkmax + 1))
     t1 = time.perf_counter()
-    ptt, backend = _engine_steps(bath, params, n, table, steps, dim)
-    t2 = time.perf_counter()
-    ref = _oracle_steps(dim, table, n, dkmax, spec["epsrel"], steps)
-    t3 = time.perf_counter()
+    ptt = oqupy.PtTempo(bath, 0.0, n * params.dt + 1e-9 * params.dt,
+                        params, backend_config={"svd_log": LOG_VALUES})
+    assert ptt.num_steps == n
+    backend = ptt.backend
+    backend._influence = lambda dk: table.get(dk)
+    ref = PtTempoOracle(dim, lambda dk: table.get(dk), np.ones(dim * dim),
+                        n, dkmax, epsrel)
+    ref.spectra = []
+    steer = models.SteeredCut([], epsrel, SPECTRUM_TOL)
+    t_engine = t_oracle = 0.0
+    ta = time.perf_counter()
+    backend.initialize()
+    tb = time.perf_counter()
+    steer.engine_log = backend.get_svd_log()
+    with steer:
+        ref.initialize()
+    tc = time.perf_counter()
+    t_engine += tb - ta
+    t_oracle += tc - tb
+    for _ in range(steps):
+        state = _export(backend)
+        ta = time.perf_counter()
+        backend.compute_step()
+        tb = time.perf_counter()
+        steer.engine_log = backend.get_svd_log()
+        ref.load_state(*state)
+        with steer:
+            ref.compute_step()
+        tc = time.perf_counter()
+        t_engine += tb - ta
+        t_oracle += tc - tb
     log = backend.get_svd_log()
-    worst = models.compare_svd_logs(log, ref.spectra, spec["epsrel"],
-                                    SPECTRUM_TOL)
+    worst = models.compare_svd_logs(log, ref.spectra, epsrel, SPECTRUM_TOL,
+                                    steer.adopted)
     assert backend.bond_dimensions() == ref.bond_dimensions()
     print(f"  {key}: {len(log)} SVDs, max bond {max(ref.bond_dimensions())}, "
-          f"max |ds|/s0 = {worst:.2e}; influence {t1 - t0:.1f} s, engine "
-          f"{t2 - t1:.1f} s, oracle {t3 - t2:.1f} s", flush=True)
+          f"max |ds|/s0 = {worst:.2e}, rounding-ambiguous cuts "
+          f"{len(steer.adopted)}; influence {t1 - t0:.1f} s, engine "
+          f"{t_engine:.1f} s, oracle {t_oracle:.1f} s", flush=True)
     backend.release()
 
 

@@ -178,3 +178,52 @@ def test_unique_matches_full_network(form, dkmax, n, epsrel, tol):
         red, _ = oracle_pt_states(system, bath, params, rho0, n, unique=True)
     err = np.abs(full - red).max()
     assert err < tol, err
+
+
+def test_rounding_ambiguous_cut_rule():
+    """The parity rule for a kept rank that differs from the oracle's: only
+    a rank the cut reaches with the spectrum moved by rel_tol s_0."""
+    from oracle.chain import truncation_rank
+    eps, tol = 1e-3, 1e-10
+    # tail norm from index 2 on is exactly at the threshold (up to rounding)
+    s = np.array([1.0, 0.5, eps * 0.8, eps * 0.6])
+    assert truncation_rank(s, eps) in (2, 3)
+    assert [k2 for k2 in range(5)
+            if models.admissible_rank(s, k2, eps, tol)] == [2, 3]
+    # a clear gap: only the rule's own rank
+    s = np.array([1.0, 0.5, 1e-2, 1e-6, 1e-7])
+    k = truncation_rank(s, eps)
+    assert k == 3
+    assert [k2 for k2 in range(6)
+            if models.admissible_rank(s, k2, eps, tol)] == [3]
+
+
+def test_steered_cut_adopts_only_ambiguous_ranks():
+    """SteeredCut on an oracle run: the oracle's own log changes nothing; a
+    log whose rank is off by one at a cut with a clear gap is not adopted,
+    so the comparison fails."""
+    system, bath, params = models.spin_boson(0.3, 3.0, 0.0, 0.05, 4, 1e-4)
+    infl = models.CachedInfluence(bath, params)
+    ref = PtTempoOracle(2, infl, np.ones(4), 6, 4, 1e-4)
+    ref.spectra = []
+    ref.compute()
+    from oracle.chain import truncation_rank
+    own = [(m, n, s[:truncation_rank(s, 1e-4)]) for m, n, s in ref.spectra]
+    again = PtTempoOracle(2, infl, np.ones(4), 6, 4, 1e-4)
+    again.spectra = []
+    with models.SteeredCut(own, 1e-4, 1e-10) as steer:
+        again.compute()
+    assert steer.adopted == []
+    models.compare_svd_logs(own, again.spectra, 1e-4)
+    # plant a rank one short at the first SVD that keeps more than one value
+    i = next(j for j, (_, _, kept) in enumerate(own) if len(kept) > 1)
+    slipped = list(own)
+    slipped[i] = (own[i][0], own[i][1], own[i][2][:-1])
+    third = PtTempoOracle(2, infl, np.ones(4), 6, 4, 1e-4)
+    third.spectra = []
+    with models.SteeredCut(slipped, 1e-4, 1e-10) as steer:
+        third.compute()
+    assert i not in steer.adopted
+    with pytest.raises(AssertionError):
+        models.compare_svd_logs(slipped, third.spectra, 1e-4,
+                                adopted=steer.adopted)
