@@ -36,6 +36,21 @@ __device__ unsigned long long g_phase[16];
 #define PHASE(k) (void)0
 #endif
 
+// Optional per-round timestamps of the Jacobi (thread 0 of every workgroup,
+// s_memrealtime ticks, for the first kRtRounds rounds and kRtBlocks
+// workgroups), compiled in with -DPTE_ROUND_TRACE only.
+#ifdef PTE_ROUND_TRACE
+constexpr int kRtRounds = 1024, kRtBlocks = 512, kRtMarks = 8;
+__device__ unsigned long long g_rtrace[(long)kRtRounds * kRtBlocks * kRtMarks];
+#define RTRACE(k)                                                                     \
+  do {                                                                                \
+    if (threadIdx.x == 0 && rt_round_ < kRtRounds && bid < kRtBlocks)                 \
+      g_rtrace[((long)rt_round_ * kRtBlocks + bid) * kRtMarks + (k)] = wall_clock64(); \
+  } while (0)
+#else
+#define RTRACE(k) (void)0
+#endif
+
 // ---------------------------------------------------------------------------
 // complex GEMM:  C(i,j) = alpha * sum_k opA(A)(i,k) * opB(B)(k,j) [+ C(i,j)]
 // 64x64 output tile per 256-thread workgroup, 4x4 outputs per thread, K in
@@ -561,6 +576,10 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
   for (; sweep < a.max_sweeps; ++sweep) {
     for (int round = 0; round < a.nb - 1; ++round) {
       PHASE(7);
+#ifdef PTE_ROUND_TRACE
+      const int rt_round_ = sweep * (a.nb - 1) + round;
+#endif
+      RTRACE(0);
       int bi, bj;
       tournament_pair(a.nb, round, pair, bi, bj);
       if (tid < JP) {
@@ -579,6 +598,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         skip_pair = ok != 0u && ld_agent(a.skip + bi) < ok && ld_agent(a.skip + bj) < ok;
       }
       __syncthreads();
+      RTRACE(1);
       if (skip_pair) {
         // the pair barrier is kept in a skipped round: without it a slice
         // could finish the next round before a slower slice of its pair
@@ -628,8 +648,10 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
 #pragma unroll
           for (int v = 0; v < 2; ++v) st_agent(mine + ((ti + 16 * u) * JP + tj + 16 * v), g[u][v]);
         PHASE(0);
+        RTRACE(2);
         grid_barrier(pair_bar + pair, R, pair_epoch);
         PHASE(1);
+        RTRACE(3);
         // identical summation order (slice 0, 1, ...) in every workgroup of
         // the pair; loads of four slices are in flight together
         double2 acc[4];
@@ -675,6 +697,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
       }
       __syncthreads();
       PHASE(2);
+      RTRACE(4);
       // record the outcome for later rounds: a rotated pair's blocks change,
       // an unrotated pair is converged as long as they do not
       if (slice == 0 && tid == 0) {
@@ -760,6 +783,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
           if (!__syncthreads_or(any)) break;
         }
         PHASE(3);
+        RTRACE(5);
         // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws over this slice ----
         const int rr = tid >> 3, c0 = (tid & 7) * 4;
         const int nx = (x_hi - x_lo + JR - 1) / JR, nv = (v_hi - v_lo + JR - 1) / JR;
@@ -804,6 +828,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         }
         if (tid == 0) st_agent(a.flags + sweep, 1);
         PHASE(4);
+        RTRACE(6);
       }
       }   // skip_pair
       // the last slice of the pair to finish releases both blocks
@@ -819,6 +844,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
       }
       ++gen;
       PHASE(5);
+      RTRACE(7);
     }
     grid_barrier(a.bar, nblk, epoch);
     if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED,
@@ -1819,6 +1845,23 @@ identity_carry_kernel(double2* dst, int R, int V) {
 }
 
 }  // namespace pte
+
+#ifdef PTE_ROUND_TRACE
+// per-round timestamp readout (n entries at most); clears the buffer
+extern "C" int pte_debug_round_trace(unsigned long long* out, long n) {
+  const long total = (long)pte::kRtRounds * pte::kRtBlocks * pte::kRtMarks;
+  if (n > total) n = total;
+  if (hipDeviceSynchronize() != hipSuccess) return -3;
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(pte::g_rtrace), sizeof(unsigned long long) * n) !=
+      hipSuccess)
+    return -3;
+  void* dev = nullptr;
+  if (hipGetSymbolAddress(&dev, HIP_SYMBOL(pte::g_rtrace)) != hipSuccess) return -3;
+  if (hipMemset(dev, 0, sizeof(unsigned long long) * total) != hipSuccess) return -3;
+  if (hipDeviceSynchronize() != hipSuccess) return -3;
+  return 0;
+}
+#endif
 
 #ifdef PTE_PHASE_TIMING
 // phase timer readout (s_memrealtime ticks); resets the timers
