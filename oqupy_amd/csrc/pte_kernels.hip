@@ -430,6 +430,11 @@ constexpr int JB = 16;          // block width
 constexpr int JP = 2 * JB;      // columns per block pair
 constexpr int JR = 32;          // rows per streamed chunk
 constexpr int JT = 256;         // threads per workgroup
+#ifndef PTE_JACOBI_SUM_BATCH
+#define PTE_JACOBI_SUM_BATCH 4
+#endif
+// slices of partial Grams whose loads are in flight together
+constexpr int kJacobiSumBatch = PTE_JACOBI_SUM_BATCH;
 
 __device__ __forceinline__ void tournament_pair(int nplayers, int round, int slot,
                                                 int& a, int& b) {
@@ -496,6 +501,47 @@ __device__ __forceinline__ bool jacobi_rotation(double app, double aqq, double2 
   return true;
 }
 
+// The same rotation with two dependent square-root chains instead of four:
+// with d = aqq - app and h = sqrt(d^2 + 4|apq|^2), cos(theta) =
+// sqrt((h + |d|) / (2h)) and sin(theta) = sign(d) |apq| / (h cos(theta))
+// (theta = pi/4 for d = 0); no sum in it cancels. The phase apq / |apq| is an
+// independent chain. The 2x2 block is first scaled by a power of two that
+// brings max(app, aqq) into [1, 2) (the rotation is scale-invariant and
+// |apq|^2 <= app aqq keeps every square finite), and everything is evaluated
+// without branches (a pair that needs no rotation runs the formula on
+// harmless operands and keeps the identity). r is the same
+// {cs, sn, Re ph, Im ph} as jacobi_rotation's up to rounding.
+__device__ __forceinline__ bool jacobi_rotation_fast(bool valid, double app, double aqq,
+                                                     double2 apq, double tol2, double abs_tol2,
+                                                     double* r) {
+  const double mag2 = cabs2(apq);
+  // needs_rotation, without short-circuit branches
+  const bool need = valid & (mag2 > tol2 * fabs(app * aqq)) &
+                    (mag2 > abs_tol2 * fmax(fmax(app, aqq), 0.0)) & (mag2 > 0.0);
+  const double sc = need ? ldexp(1.0, -ilogb(fmax(fmax(app, aqq), 1e-300))) : 0.0;
+  const double2 c = make_double2(apq.x * sc, apq.y * sc);
+  const double m2 = need ? cabs2(c) : 1.0;
+  const double d = (aqq - app) * sc;
+  const double h2 = fma(d, d, 4.0 * m2);
+  // 1 / |apq| and 1 / h: hardware estimates and two Newton steps each
+  double inv = __builtin_amdgcn_rsq(m2), rh = __builtin_amdgcn_rsq(h2);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    inv = inv * fma(-0.5 * m2 * inv, inv, 1.5);
+    rh = rh * fma(-0.5 * h2 * rh, rh, 1.5);
+  }
+  const double cos2 = fma(0.5 * fabs(d), rh, 0.5);   // cos^2(theta)
+  double rc = __builtin_amdgcn_rsq(cos2);             // 1 / cos(theta)
+#pragma unroll
+  for (int it = 0; it < 2; ++it) rc = rc * fma(-0.5 * cos2 * rc, rc, 1.5);
+  const double sn = copysign(m2 * inv * rh * rc, d < 0.0 ? -1.0 : 1.0);
+  r[0] = need ? cos2 * rc : 1.0;
+  r[1] = need ? sn : 0.0;
+  r[2] = need ? c.x * inv : 1.0;
+  r[3] = need ? c.y * inv : 0.0;
+  return need;
+}
+
 // One 32-row chunk of the pair's 32 columns (rows r0.., below hi), four
 // entries per thread (entry e = tid + 256 q: row e % 32, column e / 32), read
 // with agent-scope loads; PF chunks are issued before any is consumed so
@@ -525,8 +571,12 @@ __device__ __forceinline__ void put_chunk(double2 (*Xs)[JP + 1], const double2 (
 __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
                                             const unsigned nblk) {
   __shared__ double2 Xs[JR][JP + 1];
-  __shared__ double2 Gs[2][JP][JP + 1];
-  __shared__ double2 Ws[JP][JP + 1];
+  // rows padded by two entries: the sub-steps' scattered 16-byte accesses
+  // conflict less in the LDS banks than with one
+  __shared__ double2 Gs[2][JP][JP + 2];
+  __shared__ double2 Ws[JP][JP + 2];
+  // the pairs (p | q << 8, p < q) of every sub-step of the inner sweep
+  __shared__ int ptab[JP - 1][JB];
   __shared__ int cols[JP];
   __shared__ int need;
   __shared__ bool skip_pair;
@@ -538,6 +588,11 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
   const int x_lo = min(a.m, slice * a.xrows_per), x_hi = min(a.m, x_lo + a.xrows_per);
   const int v_lo = min(a.n, slice * a.vrows_per), v_hi = min(a.n, v_lo + a.vrows_per);
   unsigned epoch = 0;
+  for (int e = tid; e < (JP - 1) * JB; e += JT) {
+    int p, q;
+    ordered_pair(e / JB, e % JB, p, q);
+    ptab[e / JB][e % JB] = p | (q << 8);
+  }
 
   // V = identity, Frobenius norm of X
   for (long e = bid * (long)JT + tid; e < (long)a.n * a.n;
@@ -653,21 +708,22 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         PHASE(1);
         RTRACE(3);
         // identical summation order (slice 0, 1, ...) in every workgroup of
-        // the pair; loads of four slices are in flight together
+        // the pair; loads of eight slices are in flight together
+        constexpr int SB = kJacobiSumBatch;
         double2 acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
         const double2* gp = gbuf + (long)pair * R * (JP * JP);
-        for (int sl0 = 0; sl0 < R; sl0 += 4) {
-          double2 v[4][4];
+        for (int sl0 = 0; sl0 < R; sl0 += SB) {
+          double2 v[SB][4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
+          for (int t = 0; t < SB; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
               v[t][q] = (sl0 + t < R) ? ld_agent(gp + (long)(sl0 + t) * (JP * JP) + tid + JT * q)
                                       : make_double2(0.0, 0.0);
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
+          for (int t = 0; t < SB; ++t)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
               if (sl0 + t < R) acc[q] = cadd(acc[q], v[t][q]);
@@ -717,26 +773,38 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         // computes the two rotations it needs itself (identical in every
         // thread); it also rotates rows P1 and P1+16 of Ws for pair P2.
         const int P1 = tid >> 4, P2 = tid & 15;
+        // valid columns of the pair (the last block may be partial)
+        const unsigned vmask =
+            (bi * JB + JB <= a.n ? 0xffffu : (bi * JB < a.n ? (1u << (a.n - bi * JB)) - 1u : 0u)) |
+            ((bj * JB + JB <= a.n ? 0xffffu : (bj * JB < a.n ? (1u << (a.n - bj * JB)) - 1u : 0u))
+             << JB);
         int cur = 0;
         for (int isw = 0; isw < a.inner_sweeps; ++isw) {
           int any = 0;
           for (int sub = 0; sub < JP - 1; ++sub) {
-            int p1, q1, p2, q2;
-            ordered_pair(sub, P1, p1, q1);
-            ordered_pair(sub, P2, p2, q2);
-            // every lane computes the rotation of pair P2 = lane & 15; the one
-            // of pair P1 comes from lane P1 of the same wavefront
-            double r1[4], r2[4];
-            const bool on2 = (cols[p2] >= 0 && cols[q2] >= 0) &&
-                jacobi_rotation(Gs[cur][p2][p2].x, Gs[cur][q2][q2].x, Gs[cur][p2][q2], tol2,
-                                abs_tol2, r2);
-#pragma unroll
-            for (int h = 0; h < 4; ++h) r1[h] = __shfl(r2[h], P1, 64);
-            const bool on1 = __shfl((int)on2, P1, 64) != 0;
-            any |= (on1 || on2);
+            const int e1 = ptab[sub][P1], e2 = ptab[sub][P2];
+            const int p1 = e1 & 255, q1 = e1 >> 8, p2 = e2 & 255, q2 = e2 >> 8;
+            // every LDS operand of the sub-step is read at once; each lane
+            // computes the rotation of its pair P2 = lane & 15 and takes the
+            // one of pair P1 from lane P1 of its wavefront
+            const double2 g22 = Gs[cur][p2][q2];
+            const double d2p = Gs[cur][p2][p2].x, d2q = Gs[cur][q2][q2].x;
             double2 b00 = Gs[cur][p1][p2], b01 = Gs[cur][p1][q2];
             double2 b10 = Gs[cur][q1][p2], b11 = Gs[cur][q1][q2];
-            if (on2) {
+            double2 w0p = Ws[P1][p2], w0q = Ws[P1][q2];
+            double2 w1p = Ws[P1 + 16][p2], w1q = Ws[P1 + 16][q2];
+            // keep every read above issued before the rotation's arithmetic
+            __builtin_amdgcn_sched_barrier(0);
+            double r1[4], r2[4];
+            const bool on2 = jacobi_rotation_fast(((vmask >> p2) & (vmask >> q2) & 1u) != 0u,
+                                                  d2p, d2q, g22, tol2, abs_tol2, r2);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) r1[h] = __shfl(r2[h], P1, 64);
+            any |= on2;
+            // applied without branches: a pair that needs no rotation has the
+            // identity (cs 1, sn 0, ph 1), which leaves its entries exactly as
+            // they are
+            {
               const double c = r2[0], sn = r2[1];
               const double2 cph = make_double2(r2[2], -r2[3]);
               const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
@@ -745,8 +813,13 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
               const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
               const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
               b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+              const double2 y0 = cmul(w0q, cph), y1 = cmul(w1q, cph);
+              Ws[P1][p2] = make_double2(c * w0p.x - sn * y0.x, c * w0p.y - sn * y0.y);
+              Ws[P1][q2] = make_double2(sn * w0p.x + c * y0.x, sn * w0p.y + c * y0.y);
+              Ws[P1 + 16][p2] = make_double2(c * w1p.x - sn * y1.x, c * w1p.y - sn * y1.y);
+              Ws[P1 + 16][q2] = make_double2(sn * w1p.x + c * y1.x, sn * w1p.y + c * y1.y);
             }
-            if (on1) {
+            {
               const double c = r1[0], sn = r1[1];
               const double2 ph = make_double2(r1[2], r1[3]);
               const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
@@ -756,7 +829,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
               const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
               b00 = n00; b01 = n01; b10 = n10; b11 = n11;
             }
-            if (P1 == P2 && on1) {
+            if (P1 == P2 && on2) {
               b01 = make_double2(0.0, 0.0);
               b10 = make_double2(0.0, 0.0);
               b00.y = 0.0;
@@ -765,18 +838,6 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
             const int nxt = cur ^ 1;
             Gs[nxt][p1][p2] = b00; Gs[nxt][p1][q2] = b01;
             Gs[nxt][q1][p2] = b10; Gs[nxt][q1][q2] = b11;
-            if (on2) {
-              const double c = r2[0], sn = r2[1];
-              const double2 cph = make_double2(r2[2], -r2[3]);
-#pragma unroll
-              for (int h = 0; h < 2; ++h) {
-                const int row = P1 + 16 * h;
-                const double2 wp = Ws[row][p2];
-                const double2 wq = cmul(Ws[row][q2], cph);
-                Ws[row][p2] = make_double2(c * wp.x - sn * wq.x, c * wp.y - sn * wq.y);
-                Ws[row][q2] = make_double2(sn * wp.x + c * wq.x, sn * wp.y + c * wq.y);
-              }
-            }
             __syncthreads();
             cur = nxt;
           }
@@ -816,14 +877,19 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
               for (int cc = 0; cc < 4; ++cc) cfma(o[cc], xv, Ws[i][c0 + cc]);
             }
             __syncthreads();
-            const int gr = r0 + rr;
-            if (gr < hi) {
+            // through LDS, so that each store covers 32 consecutive rows of
+            // one column (512 contiguous bytes) instead of scattered entries
 #pragma unroll
-              for (int cc = 0; cc < 4; ++cc) {
-                const int gc = cols[c0 + cc];
-                if (gc >= 0) st_agent(base + (gr + (long)gc * ld), o[cc]);
-              }
+            for (int cc = 0; cc < 4; ++cc) Xs[rr][c0 + cc] = o[cc];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int e = tid + JT * q;
+              const int row = e % JR, col = e / JR;
+              const int gr = r0 + row, gc = cols[col];
+              if (gr < hi && gc >= 0) st_agent(base + (gr + (long)gc * ld), Xs[row][col]);
             }
+            __syncthreads();
           }
         }
         if (tid == 0) st_agent(a.flags + sweep, 1);
