@@ -288,6 +288,12 @@ struct Workspace {
   int jacobi_sync_words = kBarrierWords;   // words of `bar` the last prepared Jacobi uses
   int qr_blocks = 256;       // tuning: PTE_QR_BLOCKS (workgroups of the pivoted QR)
   int slice_rows = 64;       // tuning: PTE_JACOBI_SLICE_ROWS (min rows per slice)
+  // LDS-tile Jacobi (one workgroup per compute unit): enabled unless
+  // PTE_JACOBI_TILE=0; slices keep at least tile_rows rows of X
+  // (PTE_JACOBI_TILE_ROWS); a slice's X and V rows together fit kTileMaxRows
+  bool tile_enabled = true;
+  int tile_rows = 32;
+  int tile_coop_blocks = 0;
   bool small_enabled = true; // tuning: PTE_SMALL_SVD=0 sends every SVD to the grid path
   int max_coop_blocks = 0;
   double epsrel = 1e-7;
@@ -356,6 +362,7 @@ struct Workspace {
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     max_coop_blocks = per_cu * prop.multiProcessorCount;
     num_cus = prop.multiProcessorCount;
+    init_tile_kernel();
     HIP_CHECK(hipMalloc((void**)&qr_cand, sizeof(QrcpCand) * 2 * 256));
     HIP_CHECK(hipMalloc((void**)&qr_rank_d, sizeof(int)));
     HIP_CHECK(hipHostMalloc((void**)&qr_rank_h, sizeof(int), hipHostMallocDefault));
@@ -456,6 +463,7 @@ struct Workspace {
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     max_coop_blocks = per_cu * prop.multiProcessorCount;
     num_cus = prop.multiProcessorCount;
+    init_tile_kernel();
     require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
     HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -502,9 +510,43 @@ struct Workspace {
   // ---- phases of the truncated SVD, shared by the single and the batched
   // launches (svd_stream is the stream they are enqueued on) ----
 
+  // Launch limits of the LDS-tile Jacobi (its LDS allows one workgroup per
+  // compute unit) and its tuning variables.
+  void init_tile_kernel() {
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_tile_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)tile_lds_bytes(kTileMaxRows)));
+    int per_cu = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(jacobi_tile_kernel), 256,
+        tile_lds_bytes(kTileMaxRows)));
+    tile_coop_blocks = per_cu * num_cus;
+    if (const char* v = std::getenv("PTE_JACOBI_TILE")) tile_enabled = std::atoi(v) != 0;
+    if (const char* v = std::getenv("PTE_JACOBI_TILE_ROWS"))
+      tile_rows = std::max(32, std::min(kTileMaxRows, std::atoi(v)));
+  }
+
+  static size_t tile_lds_bytes(int rows) {
+    return (size_t)rows * kJacobiTileLd * sizeof(double2);
+  }
+
+  // Row slices of the LDS-tile Jacobi on X (m x n): the most slices (up to
+  // 16, at least tile_rows rows of X each) that one co-resident grid holds;
+  // 0 when the path is off or a slice's rows do not fit the tile.
+  int jacobi_tile_slices(int m, int n) const {
+    if (!tile_enabled || tile_coop_blocks <= 0) return 0;
+    const int npairs = jacobi_blocks(n) / 2;
+    if (npairs > tile_coop_blocks) return 0;
+    const int R = std::max(1, std::min({tile_coop_blocks / npairs,
+                                        (m + tile_rows - 1) / tile_rows, 16}));
+    auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
+    const int rows = round32((m + R - 1) / R) + round32((n + R - 1) / R);
+    return rows <= kTileMaxRows ? R : 0;
+  }
+
   // Buffers and arguments of the Jacobi on X (m x n, column-major, ld = m);
   // at most max_grid workgroups. *grid_out receives the grid size.
-  JacobiArgs jacobi_prepare(int m, int n, int max_grid, int* grid_out) {
+  JacobiArgs jacobi_prepare(int m, int n, int max_grid, int* grid_out, int slices = 0) {
     require(m >= 1 && n >= 1, PTE_ERR_ARG, "internal: empty SVD");
     V.ensure((size_t)n * n, svd_stream);
     ensure_sv((size_t)n);
@@ -516,8 +558,9 @@ struct Workspace {
             "SVD of " + std::to_string(n) + " columns exceeds the co-resident grid");
     // row slices per block pair: each slice keeps at least slice_rows rows of
     // X, at most 16 slices, within the grid
-    const int R = std::max(1, std::min({max_grid / npairs,
-                                        (m + slice_rows - 1) / slice_rows, 16}));
+    const int R = slices > 0 ? slices
+                             : std::max(1, std::min({max_grid / npairs,
+                                                     (m + slice_rows - 1) / slice_rows, 16}));
     auto round32 = [](int v) { return ((v + 31) / 32) * 32; };
     const int xrows = round32((m + R - 1) / R), vrows = round32((n + R - 1) / R);
     if (R > 1) gpart.ensure((size_t)2 * npairs * R * 1024, svd_stream);
@@ -625,12 +668,19 @@ struct Workspace {
   int svd(int m, int n) {
     if (batcher) return batched_svd(false, m, n);
     int grid = 0;
-    JacobiArgs ja = jacobi_prepare(m, n, std::min(max_coop_blocks, 1024), &grid);
+    const int tile_slices = jacobi_tile_slices(m, n);
+    JacobiArgs ja = jacobi_prepare(m, n, std::min(max_coop_blocks, 1024), &grid, tile_slices);
     jacobi_reset();
     void* kargs[] = {&ja};
     mark(evj0);
-    HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
-                                         dim3(grid), dim3(256), kargs, 0, svd_stream));
+    if (tile_slices > 0) {
+      HIP_CHECK(hipLaunchCooperativeKernel(
+          reinterpret_cast<const void*>(jacobi_tile_kernel), dim3(grid), dim3(256), kargs,
+          tile_lds_bytes(ja.xrows_per + ja.vrows_per), svd_stream));
+    } else {
+      HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(jacobi_kernel),
+                                           dim3(grid), dim3(256), kargs, 0, svd_stream));
+    }
     mark(evj1);
     jacobi_finish(m, n);
     HIP_CHECK(hipStreamSynchronize(svd_stream));

@@ -168,6 +168,9 @@ __global__ void cgemm_mfma_kernel(GemmArgs a);
 __global__ void mpo_apply_kernel(MpoApplyArgs a);
 __global__ void mpo_apply_right_kernel(MpoApplyArgs a);
 __global__ void jacobi_kernel(JacobiArgs a);
+// the same with the slice held in an LDS tile of xrows_per + vrows_per rows
+// (dynamic LDS: rows * kJacobiTileLd complex entries; at most kTileMaxRows rows)
+__global__ void jacobi_tile_kernel(JacobiArgs a);
 __global__ void jacobi_batch_kernel(const JacobiArgs* probs, const int* starts,
                                     const int* block_prob);
 __global__ void column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma);
@@ -183,5 +186,7 @@ __global__ void leg_vector_kernel(const double2* T, int A, int P, int B, const d
 __global__ void identity_carry_kernel(double2* dst, int R, int V);
 
 constexpr int kJacobiBlock = 16;
+constexpr int kJacobiTileLd = 2 * kJacobiBlock + 1;
+constexpr int kTileMaxRows = 192;
 
 }  // namespace pte

@@ -542,6 +542,93 @@ __device__ __forceinline__ bool jacobi_rotation_fast(bool valid, double app, dou
   return need;
 }
 
+// Valid columns of the block pair (bi, bj) as a 32-bit mask (the last block
+// may be partial).
+__device__ __forceinline__ unsigned pair_valid_mask(int bi, int bj, int n) {
+  const unsigned mi = bi * JB + JB <= n ? 0xffffu : (bi * JB < n ? (1u << (n - bi * JB)) - 1u : 0u);
+  const unsigned mj = bj * JB + JB <= n ? 0xffffu : (bj * JB < n ? (1u << (n - bj * JB)) - 1u : 0u);
+  return mi | (mj << JB);
+}
+
+// Cyclic two-sided Jacobi sweeps on the pair's 2JB x 2JB Gram Gs[0],
+// accumulating the rotations in Ws (identity on entry). Gs is
+// double-buffered: a sub-step reads Gs[cur] and writes every entry of
+// Gs[cur ^ 1], so one barrier per sub-step suffices. Each thread owns the
+// 2x2 block (P1, P2) of the sub-step's pairing (ptab); each lane computes the
+// rotation of pair P2 = lane & 15 and takes the one of pair P1 from lane P1
+// of its wavefront; it also rotates rows P1 and P1+16 of Ws for pair P2.
+__device__ __forceinline__ void pair_inner_sweeps(double2 (*Gs)[JP][JP + 2],
+                                                  double2 (*Ws)[JP + 2],
+                                                  const int (*ptab)[JB], unsigned vmask,
+                                                  double tol2, double abs_tol2,
+                                                  int inner_sweeps) {
+  const int tid = threadIdx.x;
+  const int P1 = tid >> 4, P2 = tid & 15;
+  int cur = 0;
+  for (int isw = 0; isw < inner_sweeps; ++isw) {
+    int any = 0;
+    for (int sub = 0; sub < JP - 1; ++sub) {
+      const int e1 = ptab[sub][P1], e2 = ptab[sub][P2];
+      const int p1 = e1 & 255, q1 = e1 >> 8, p2 = e2 & 255, q2 = e2 >> 8;
+      // every LDS operand of the sub-step is read at once
+      const double2 g22 = Gs[cur][p2][q2];
+      const double d2p = Gs[cur][p2][p2].x, d2q = Gs[cur][q2][q2].x;
+      double2 b00 = Gs[cur][p1][p2], b01 = Gs[cur][p1][q2];
+      double2 b10 = Gs[cur][q1][p2], b11 = Gs[cur][q1][q2];
+      double2 w0p = Ws[P1][p2], w0q = Ws[P1][q2];
+      double2 w1p = Ws[P1 + 16][p2], w1q = Ws[P1 + 16][q2];
+      // keep every read above issued before the rotation's arithmetic
+      __builtin_amdgcn_sched_barrier(0);
+      double r1[4], r2[4];
+      const bool on2 = jacobi_rotation_fast(((vmask >> p2) & (vmask >> q2) & 1u) != 0u, d2p,
+                                            d2q, g22, tol2, abs_tol2, r2);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) r1[h] = __shfl(r2[h], P1, 64);
+      any |= on2;
+      // applied without branches: a pair that needs no rotation has the
+      // identity (cs 1, sn 0, ph 1), which leaves its entries exactly as
+      // they are
+      {
+        const double c = r2[0], sn = r2[1];
+        const double2 cph = make_double2(r2[2], -r2[3]);
+        const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
+        const double2 n00 = make_double2(c * b00.x - sn * x01.x, c * b00.y - sn * x01.y);
+        const double2 n01 = make_double2(sn * b00.x + c * x01.x, sn * b00.y + c * x01.y);
+        const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
+        const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
+        b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+        const double2 y0 = cmul(w0q, cph), y1 = cmul(w1q, cph);
+        Ws[P1][p2] = make_double2(c * w0p.x - sn * y0.x, c * w0p.y - sn * y0.y);
+        Ws[P1][q2] = make_double2(sn * w0p.x + c * y0.x, sn * w0p.y + c * y0.y);
+        Ws[P1 + 16][p2] = make_double2(c * w1p.x - sn * y1.x, c * w1p.y - sn * y1.y);
+        Ws[P1 + 16][q2] = make_double2(sn * w1p.x + c * y1.x, sn * w1p.y + c * y1.y);
+      }
+      {
+        const double c = r1[0], sn = r1[1];
+        const double2 ph = make_double2(r1[2], r1[3]);
+        const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
+        const double2 n00 = make_double2(c * b00.x - sn * y10.x, c * b00.y - sn * y10.y);
+        const double2 n01 = make_double2(c * b01.x - sn * y11.x, c * b01.y - sn * y11.y);
+        const double2 n10 = make_double2(sn * b00.x + c * y10.x, sn * b00.y + c * y10.y);
+        const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
+        b00 = n00; b01 = n01; b10 = n10; b11 = n11;
+      }
+      if (P1 == P2 && on2) {
+        b01 = make_double2(0.0, 0.0);
+        b10 = make_double2(0.0, 0.0);
+        b00.y = 0.0;
+        b11.y = 0.0;
+      }
+      const int nxt = cur ^ 1;
+      Gs[nxt][p1][p2] = b00; Gs[nxt][p1][q2] = b01;
+      Gs[nxt][q1][p2] = b10; Gs[nxt][q1][q2] = b11;
+      __syncthreads();
+      cur = nxt;
+    }
+    if (!__syncthreads_or(any)) break;
+  }
+}
+
 // One 32-row chunk of the pair's 32 columns (rows r0.., below hi), four
 // entries per thread (entry e = tid + 256 q: row e % 32, column e / 32), read
 // with agent-scope loads; PF chunks are issued before any is consumed so
@@ -766,83 +853,8 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
         }
       }
       if (need) {
-        // ---- cyclic two-sided Jacobi on Gs, accumulating Ws ----
-        // Gs is double-buffered: a sub-step reads Gs[cur] and writes every
-        // entry of Gs[cur ^ 1], so one barrier per sub-step suffices. Each
-        // thread owns the 2x2 block (P1, P2) of the sub-step's pairing and
-        // computes the two rotations it needs itself (identical in every
-        // thread); it also rotates rows P1 and P1+16 of Ws for pair P2.
-        const int P1 = tid >> 4, P2 = tid & 15;
-        // valid columns of the pair (the last block may be partial)
-        const unsigned vmask =
-            (bi * JB + JB <= a.n ? 0xffffu : (bi * JB < a.n ? (1u << (a.n - bi * JB)) - 1u : 0u)) |
-            ((bj * JB + JB <= a.n ? 0xffffu : (bj * JB < a.n ? (1u << (a.n - bj * JB)) - 1u : 0u))
-             << JB);
-        int cur = 0;
-        for (int isw = 0; isw < a.inner_sweeps; ++isw) {
-          int any = 0;
-          for (int sub = 0; sub < JP - 1; ++sub) {
-            const int e1 = ptab[sub][P1], e2 = ptab[sub][P2];
-            const int p1 = e1 & 255, q1 = e1 >> 8, p2 = e2 & 255, q2 = e2 >> 8;
-            // every LDS operand of the sub-step is read at once; each lane
-            // computes the rotation of its pair P2 = lane & 15 and takes the
-            // one of pair P1 from lane P1 of its wavefront
-            const double2 g22 = Gs[cur][p2][q2];
-            const double d2p = Gs[cur][p2][p2].x, d2q = Gs[cur][q2][q2].x;
-            double2 b00 = Gs[cur][p1][p2], b01 = Gs[cur][p1][q2];
-            double2 b10 = Gs[cur][q1][p2], b11 = Gs[cur][q1][q2];
-            double2 w0p = Ws[P1][p2], w0q = Ws[P1][q2];
-            double2 w1p = Ws[P1 + 16][p2], w1q = Ws[P1 + 16][q2];
-            // keep every read above issued before the rotation's arithmetic
-            __builtin_amdgcn_sched_barrier(0);
-            double r1[4], r2[4];
-            const bool on2 = jacobi_rotation_fast(((vmask >> p2) & (vmask >> q2) & 1u) != 0u,
-                                                  d2p, d2q, g22, tol2, abs_tol2, r2);
-#pragma unroll
-            for (int h = 0; h < 4; ++h) r1[h] = __shfl(r2[h], P1, 64);
-            any |= on2;
-            // applied without branches: a pair that needs no rotation has the
-            // identity (cs 1, sn 0, ph 1), which leaves its entries exactly as
-            // they are
-            {
-              const double c = r2[0], sn = r2[1];
-              const double2 cph = make_double2(r2[2], -r2[3]);
-              const double2 x01 = cmul(b01, cph), x11 = cmul(b11, cph);
-              const double2 n00 = make_double2(c * b00.x - sn * x01.x, c * b00.y - sn * x01.y);
-              const double2 n01 = make_double2(sn * b00.x + c * x01.x, sn * b00.y + c * x01.y);
-              const double2 n10 = make_double2(c * b10.x - sn * x11.x, c * b10.y - sn * x11.y);
-              const double2 n11 = make_double2(sn * b10.x + c * x11.x, sn * b10.y + c * x11.y);
-              b00 = n00; b01 = n01; b10 = n10; b11 = n11;
-              const double2 y0 = cmul(w0q, cph), y1 = cmul(w1q, cph);
-              Ws[P1][p2] = make_double2(c * w0p.x - sn * y0.x, c * w0p.y - sn * y0.y);
-              Ws[P1][q2] = make_double2(sn * w0p.x + c * y0.x, sn * w0p.y + c * y0.y);
-              Ws[P1 + 16][p2] = make_double2(c * w1p.x - sn * y1.x, c * w1p.y - sn * y1.y);
-              Ws[P1 + 16][q2] = make_double2(sn * w1p.x + c * y1.x, sn * w1p.y + c * y1.y);
-            }
-            {
-              const double c = r1[0], sn = r1[1];
-              const double2 ph = make_double2(r1[2], r1[3]);
-              const double2 y10 = cmul(ph, b10), y11 = cmul(ph, b11);
-              const double2 n00 = make_double2(c * b00.x - sn * y10.x, c * b00.y - sn * y10.y);
-              const double2 n01 = make_double2(c * b01.x - sn * y11.x, c * b01.y - sn * y11.y);
-              const double2 n10 = make_double2(sn * b00.x + c * y10.x, sn * b00.y + c * y10.y);
-              const double2 n11 = make_double2(sn * b01.x + c * y11.x, sn * b01.y + c * y11.y);
-              b00 = n00; b01 = n01; b10 = n10; b11 = n11;
-            }
-            if (P1 == P2 && on2) {
-              b01 = make_double2(0.0, 0.0);
-              b10 = make_double2(0.0, 0.0);
-              b00.y = 0.0;
-              b11.y = 0.0;
-            }
-            const int nxt = cur ^ 1;
-            Gs[nxt][p1][p2] = b00; Gs[nxt][p1][q2] = b01;
-            Gs[nxt][q1][p2] = b10; Gs[nxt][q1][q2] = b11;
-            __syncthreads();
-            cur = nxt;
-          }
-          if (!__syncthreads_or(any)) break;
-        }
+        const unsigned vmask = pair_valid_mask(bi, bj, a.n);
+        pair_inner_sweeps(Gs, Ws, ptab, vmask, tol2, abs_tol2, a.inner_sweeps);
         PHASE(3);
         RTRACE(5);
         // ---- X_pair <- X_pair Ws, V_pair <- V_pair Ws over this slice ----
@@ -936,6 +948,301 @@ jacobi_batch_kernel(const JacobiArgs* probs, const int* starts, const int* block
   const int s0 = starts[p];
   const JacobiArgs a = probs[p];
   jacobi_body(a, blockIdx.x - s0, starts[p + 1] - s0);
+}
+
+// The same Jacobi with the slice kept in LDS for the round (one workgroup
+// per compute unit): each round loads the slice's rows of X and of V in the
+// pair's 32 columns into an LDS tile with one batch of coalesced loads,
+// computes the partial Gram from it and, after the pair's inner sweeps, the
+// rotation X <- X W, V <- V W in place on the FP64 matrix cores
+// (v_mfma_f64_16x16x4_f64, 3M split), and writes the tile back with coalesced
+// stores. The round protocol (block versions, converged-pair skipping, pair
+// barrier, release count) is the one of jacobi_body. The tile has
+// a.xrows_per + a.vrows_per rows of JP + 1 entries (dynamic LDS).
+constexpr int kTileLd = kJacobiTileLd;
+static_assert(kTileLd == JP + 1, "tile row stride");
+
+__device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int bid,
+                                                 const unsigned nblk) {
+  extern __shared__ double2 tile[];
+  __shared__ double2 Gs[2][JP][JP + 2];
+  __shared__ double2 Ws[JP][JP + 2];
+  __shared__ int ptab[JP - 1][JB];
+  __shared__ int cols[JP];
+  __shared__ int need;
+  __shared__ bool skip_pair;
+  __shared__ double red[JT];
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int R = a.slices;
+  const int pair = bid / R, slice = bid % R;
+  const int xr = a.xrows_per, vr = a.vrows_per, tr = xr + vr;
+  const int x_lo = min(a.m, slice * xr), x_hi = min(a.m, x_lo + xr);
+  const int v_lo = min(a.n, slice * vr), v_hi = min(a.n, v_lo + vr);
+  unsigned epoch = 0;
+  for (int e = tid; e < (JP - 1) * JB; e += JT) {
+    int p, q;
+    ordered_pair(e / JB, e % JB, p, q);
+    ptab[e / JB][e % JB] = p | (q << 8);
+  }
+
+  // V = identity, Frobenius norm of X
+  for (long e = bid * (long)JT + tid; e < (long)a.n * a.n; e += (long)nblk * JT) {
+    const long i = e % a.n, j = e / a.n;
+    st_agent(a.V + (i + j * a.ldv), make_double2(i == j ? 1.0 : 0.0, 0.0));
+  }
+  double part = 0.0;
+  for (long e = bid * (long)JT + tid; e < (long)a.m * a.n; e += (long)nblk * JT) {
+    const long i = e % a.m, j = e / a.m;
+    part += cabs2(a.X[i + j * a.ldx]);
+  }
+  red[tid] = part;
+  __syncthreads();
+  for (int s = JT / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) atomicAdd(a.fro2, red[0]);
+  grid_barrier(a.bar, nblk, epoch);
+  const double fro = sqrt(__hip_atomic_load(a.fro2, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT));
+  const double abs_tol = a.abs_tol_scale * fro;
+  const double tol2 = a.tol * a.tol, abs_tol2 = abs_tol * abs_tol;
+  const int npairs = a.nb / 2;
+  unsigned* pair_bar = a.bar + kBarrierWords;
+  unsigned* pair_done = pair_bar + npairs;
+  unsigned* blockver = pair_done + npairs;
+  unsigned gen = 0, pair_epoch = 0;
+  const int ntile = tr * JP;          // entries of the tile
+
+  int sweep = 0;
+  bool converged = false;
+  for (; sweep < a.max_sweeps; ++sweep) {
+    for (int round = 0; round < a.nb - 1; ++round) {
+#ifdef PTE_ROUND_TRACE
+      const int rt_round_ = sweep * (a.nb - 1) + round;
+#endif
+      RTRACE(0);
+      int bi, bj;
+      tournament_pair(a.nb, round, pair, bi, bj);
+      if (tid < JP) {
+        const int c = (tid < JB) ? bi * JB + tid : bj * JB + (tid - JB);
+        cols[tid] = (c < a.n) ? c : -1;
+      }
+      // both blocks released by the pairs that held them in the previous round
+      const int plo = min(bi, bj), phi = max(bi, bj);
+      unsigned* const pair_ok = a.skip + a.nb + ((long)phi * (phi - 1)) / 2 + plo;
+      if (tid == 0) {
+        while (ld_agent(blockver + bi) < gen || ld_agent(blockver + bj) < gen)
+          __builtin_amdgcn_s_sleep(1);
+        const unsigned ok = ld_agent(pair_ok);
+        skip_pair = ok != 0u && ld_agent(a.skip + bi) < ok && ld_agent(a.skip + bj) < ok;
+      }
+      __syncthreads();
+      RTRACE(1);
+      if (skip_pair) {
+        // the pair barrier is kept in a skipped round (see jacobi_body)
+        if (R > 1) grid_barrier(pair_bar + pair, R, pair_epoch);
+      } else {
+        // ---- the slice's rows of X and V into the tile: entry e is row
+        // e % tr of column e / tr, so that a wavefront reads runs of
+        // consecutive rows of one column; every load of the tile is in
+        // flight at once (at most kTileMaxRows * JP / JT per thread) ----
+        {
+          constexpr int kLoads = kTileMaxRows * JP / JT;
+          double2 v[kLoads];
+#pragma unroll
+          for (int q = 0; q < kLoads; ++q) {
+            const int e = tid + JT * q;
+            const int row = e % tr, col = e / tr;
+            v[q] = make_double2(0.0, 0.0);
+            if (e < ntile && cols[col] >= 0) {
+              if (row < xr) {
+                if (x_lo + row < x_hi) v[q] = ld_agent(a.X + (x_lo + row + (long)cols[col] * a.ldx));
+              } else if (v_lo + row - xr < v_hi) {
+                v[q] = ld_agent(a.V + (v_lo + row - xr + (long)cols[col] * a.ldv));
+              }
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < kLoads; ++q) {
+            const int e = tid + JT * q;
+            if (e < ntile) tile[(e % tr) * kTileLd + e / tr] = v[q];
+          }
+        }
+        __syncthreads();
+        // partial Grams are double-buffered by the parity of the pair
+        // barrier's epoch before this round's barrier
+        const double2* const gbuf =
+            a.gpart + (long)(pair_epoch & 1) * npairs * R * (JP * JP) + (long)pair * R * (JP * JP);
+        {
+          const int I = wave >> 1, J = wave & 1;
+          doublex4 p1 = {0.0, 0.0, 0.0, 0.0}, p2 = p1, p3 = p1;
+          for (int k4 = 0; k4 < xr; k4 += 4) {
+            const double2 x = tile[(k4 + lk) * kTileLd + 16 * I + lr];
+            const double2 y = tile[(k4 + lk) * kTileLd + 16 * J + lr];
+            const double ar = x.x, ai = -x.y;
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, y.x, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, y.y, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, y.x + y.y, p3, 0, 0, 0);
+          }
+          double2* mine = const_cast<double2*>(gbuf) + (long)slice * (JP * JP);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int gi = 16 * I + 4 * r + lk, gj = 16 * J + lr;
+            const double2 g = make_double2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
+            if (R == 1) Gs[0][gi][gj] = g;
+            else st_agent(mine + (gi * JP + gj), g);
+          }
+        }
+        if (R > 1) {
+          RTRACE(2);
+          grid_barrier(pair_bar + pair, R, pair_epoch);
+          RTRACE(3);
+          // identical summation order (slice 0, 1, ...) in every workgroup
+          // of the pair; loads of eight slices in flight together
+          constexpr int SB = 8;
+          double2 acc[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
+          const double2* gp = gbuf;
+          for (int sl0 = 0; sl0 < R; sl0 += SB) {
+            double2 v[SB][4];
+#pragma unroll
+            for (int t = 0; t < SB; ++t)
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                v[t][q] = (sl0 + t < R) ? ld_agent(gp + (long)(sl0 + t) * (JP * JP) + tid + JT * q)
+                                        : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int t = 0; t < SB; ++t)
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (sl0 + t < R) acc[q] = cadd(acc[q], v[t][q]);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int e = tid + JT * q;
+            Gs[0][e / JP][e % JP] = acc[q];
+          }
+        }
+        for (int e = tid; e < JP * JP; e += JT)
+          Ws[e / JP][e % JP] = make_double2((e / JP) == (e % JP) ? 1.0 : 0.0, 0.0);
+        if (tid == 0) need = 0;
+        __syncthreads();
+        // ---- does the pair need a rotation? ----
+        {
+          int mine = 0;
+          for (int e = tid; e < JP * JP; e += JT) {
+            const int gi = e / JP, gj = e % JP;
+            if (gi < gj && cols[gi] >= 0 && cols[gj] >= 0) {
+              if (needs_rotation(Gs[0][gi][gi].x, Gs[0][gj][gj].x, cabs2(Gs[0][gi][gj]), tol2,
+                                 abs_tol2))
+                mine = 1;
+            }
+          }
+          if (mine) need = 1;
+        }
+        __syncthreads();
+        RTRACE(4);
+        // record the outcome for later rounds: a rotated pair's blocks change,
+        // an unrotated pair is converged as long as they do not
+        if (slice == 0 && tid == 0) {
+          if (need) {
+            st_agent(a.skip + bi, gen + 1);
+            st_agent(a.skip + bj, gen + 1);
+            st_agent(pair_ok, 0u);
+          } else {
+            st_agent(pair_ok, gen + 1);
+          }
+        }
+        if (need) {
+          pair_inner_sweeps(Gs, Ws, ptab, pair_valid_mask(bi, bj, a.n), tol2, abs_tol2,
+                            a.inner_sweeps);
+          RTRACE(5);
+          // ---- tile <- tile W on the matrix cores: B = W (lane: k = 4 k4 +
+          // lk, column lr of the 16-column half v), A = 16 tile rows ----
+          double wr[8][2], wi[8][2];
+#pragma unroll
+          for (int k4 = 0; k4 < 8; ++k4)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              const double2 w = Ws[4 * k4 + lk][16 * v + lr];
+              wr[k4][v] = w.x;
+              wi[k4][v] = w.y;
+            }
+          for (int t = wave; t < tr / 16; t += 4) {
+            doublex4 p1[2], p2[2], p3[2];
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              p1[v] = doublex4{0.0, 0.0, 0.0, 0.0};
+              p2[v] = p1[v];
+              p3[v] = p1[v];
+            }
+#pragma unroll
+            for (int k4 = 0; k4 < 8; ++k4) {
+              const double2 x = tile[(16 * t + lr) * kTileLd + 4 * k4 + lk];
+#pragma unroll
+              for (int v = 0; v < 2; ++v) {
+                p1[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x, wr[k4][v], p1[v], 0, 0, 0);
+                p2[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.y, wi[k4][v], p2[v], 0, 0, 0);
+                p3[v] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.x + x.y, wr[k4][v] + wi[k4][v],
+                                                             p3[v], 0, 0, 0);
+              }
+            }
+            // the wavefront's reads of these 16 rows are complete (in-order
+            // LDS operations of one wavefront) before it overwrites them
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                tile[(16 * t + 4 * r + lk) * kTileLd + 16 * v + lr] =
+                    make_double2(p1[v][r] - p2[v][r], p3[v][r] - p1[v][r] - p2[v][r]);
+          }
+          __syncthreads();
+          // ---- the tile back, entry e as in the load ----
+          for (int e = tid; e < ntile; e += JT) {
+            const int row = e % tr, col = e / tr;
+            const int gc = cols[col];
+            if (gc < 0) continue;
+            const double2 val = tile[row * kTileLd + col];
+            if (row < xr) {
+              if (x_lo + row < x_hi) st_agent(a.X + (x_lo + row + (long)gc * a.ldx), val);
+            } else if (v_lo + row - xr < v_hi) {
+              st_agent(a.V + (v_lo + row - xr + (long)gc * a.ldv), val);
+            }
+          }
+          if (tid == 0) st_agent(a.flags + sweep, 1);
+          RTRACE(6);
+        }
+      }   // skip_pair
+      // the last slice of the pair to finish releases both blocks
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) {
+        const unsigned old = __hip_atomic_fetch_add(pair_done + pair, 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        if (old == (gen + 1) * (unsigned)R - 1) {
+          st_agent(blockver + bi, gen + 1);
+          st_agent(blockver + bj, gen + 1);
+        }
+      }
+      ++gen;
+      RTRACE(7);
+    }
+    grid_barrier(a.bar, nblk, epoch);
+    if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      converged = true;
+      break;
+    }
+  }
+  if (bid == 0 && tid == 0) *a.info = converged ? sweep + 1 : -1;
+}
+
+__global__ void __launch_bounds__(JT)
+jacobi_tile_kernel(JacobiArgs a) {
+  jacobi_tile_body(a, blockIdx.x, gridDim.x);
 }
 
 // sigma_j = || X[:, j] ||  (one workgroup per column)

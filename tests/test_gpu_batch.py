@@ -54,9 +54,16 @@ def test_batch_of_one_equals_single():
     end = 14 * 0.05 + 1e-11
     (pt_b,) = oqupy.pt_tempo_compute_batch([bath], 0.0, end, params)
     pt_s = oqupy.pt_tempo_compute(bath, 0.0, end, params)
-    for k in range(len(pt_s)):
-        assert np.abs(pt_b.get_mpo_tensor(k) - pt_s.get_mpo_tensor(k)).max() \
-            < 1e-13
+    # the batched Jacobi and the single engine's LDS-tile Jacobi round
+    # differently, so the factors agree up to the gauge of the SVDs: compare
+    # the bonds and the dynamics the two process tensors give
+    assert list(pt_b.get_bond_dimensions()) == list(pt_s.get_bond_dimensions())
+    rho0 = op.spin_dm("z+")
+    dyn_b = oqupy.compute_dynamics(system=system, process_tensor=pt_b,
+                                   initial_state=rho0)
+    dyn_s = oqupy.compute_dynamics(system=system, process_tensor=pt_s,
+                                   initial_state=rho0)
+    assert np.abs(np.asarray(dyn_b.states) - np.asarray(dyn_s.states)).max() < 1e-12
 
 
 def test_batch_rejects_unsupported():
