@@ -178,10 +178,11 @@ static std::vector<cplx> to_cplx(const double* data, long count) {
 // ---------------------------------------------------------------------------
 // workspace and the truncated SVD
 // ---------------------------------------------------------------------------
-// Longest column for the QR-preconditioned SVD: apply_q_kernel holds one
-// column of Q in LDS (16 bytes per row, below 64 KiB with its 4 KiB of
-// reduction buffers).
-constexpr int kQrMaxRows = 3584;
+// Longest column for the QR-preconditioned SVD: qrcp_kernel holds the step's
+// reflector and apply_q_kernel one column of Q in LDS, 16 bytes per row
+// (144 KiB of the 160 KiB of LDS of a compute unit, beside their reduction
+// buffers). Taller matrices take the unpreconditioned Jacobi.
+constexpr int kQrMaxRows = 9216;
 
 // largest rank a truncated SVD of an m x n matrix can keep
 static inline int k_bound(int m, int n) { return std::max(1, std::min(m, n)); }
@@ -513,6 +514,12 @@ struct Workspace {
   // Launch limits of the LDS-tile Jacobi (its LDS allows one workgroup per
   // compute unit) and its tuning variables.
   void init_tile_kernel() {
+    // LDS of the QR path: one column of up to kQrMaxRows rows
+    for (const void* f : {reinterpret_cast<const void*>(qrcp_kernel),
+                          reinterpret_cast<const void*>(qrcp_batch_kernel),
+                          reinterpret_cast<const void*>(apply_q_kernel)})
+      HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(kQrMaxRows * sizeof(double2))));
     HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_tile_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)tile_lds_bytes(kTileMaxRows)));

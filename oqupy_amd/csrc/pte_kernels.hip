@@ -1506,6 +1506,24 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
   const int j0 = a.start;
   double wmax = -1.0, wsum = 0.0;
   int wcol = -1;
+  // each wavefront keeps its first own column (q = wave) in registers for the
+  // whole factorisation, rows lane + 64 u (u < kQrCache) at absolute row
+  // indices (rows below the current step are left as they are); its
+  // updates are still stored every step, for the pivot reads and the
+  // factors. It stays resident until it is pivoted.
+  double2 res[kQrCache];
+  bool resident = false;
+#pragma unroll
+  for (int u = 0; u < kQrCache; ++u) res[u] = make_double2(0.0, 0.0);
+  if (wave < nloc && !(j0 > 0 && ld_agent(a.done + blk + wave * nblk))) {
+    const double2* col = a.A + (long)(blk + wave * nblk) * lda;
+#pragma unroll
+    for (int u = 0; u < kQrCache; ++u) {
+      const int i = lane + 64 * u;
+      if (i >= j0 && i < m) res[u] = ld_agent(col + i);
+    }
+    resident = true;
+  }
   for (int q = wave; q < nloc; q += 4) {
     const int c = blk + q * nblk;
     if (j0 > 0 && ld_agent(a.done + c)) {
@@ -1602,6 +1620,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       done[p / nblk] = 1;
       st_agent(a.done + p, 1);
     }
+    if (p % nblk == blk && p / nblk == wave) resident = false;
     __syncthreads();
     PHASE(8);
     // apply H_j^H to the own remaining columns, downdate their norms
@@ -1612,6 +1631,99 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       if (done[q]) continue;
       const int c = blk + q * nblk;
       double2* col = a.A + (long)c * lda;
+      if (q == wave && resident) {
+        // the resident column: rows below 64 kQrCache from registers, the
+        // rest (if any) streamed
+        const int beyond = max(j, 64 * kQrCache);
+        if (reflect) {
+          double2 w = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int u = 0; u < kQrCache; ++u) {
+            const int i = lane + 64 * u;
+            if (i >= j && i < m) cfmac(w, vcol[i], res[u]);
+          }
+          for (int base = beyond + lane; base < m; base += 64 * kQrBatch) {
+            double2 t[kQrBatch];
+#pragma unroll
+            for (int u = 0; u < kQrBatch; ++u) {
+              const int i = base + 64 * u;
+              t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kQrBatch; ++u) {
+              const int i = base + 64 * u;
+              if (i < m) cfmac(w, vcol[i], t[u]);
+            }
+          }
+          w.x = wave_sum(w.x);
+          w.y = wave_sum(w.y);
+          const double2 tw = cmul(tc, w);
+#pragma unroll
+          for (int u = 0; u < kQrCache; ++u) {
+            const int i = lane + 64 * u;
+            if (i >= j && i < m) {
+              res[u] = csub(res[u], cmul(vcol[i], tw));
+              st_agent(col + i, res[u]);
+            }
+          }
+          for (int base = beyond + lane; base < m; base += 64 * kQrBatch) {
+            double2 t[kQrBatch];
+#pragma unroll
+            for (int u = 0; u < kQrBatch; ++u) {
+              const int i = base + 64 * u;
+              t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kQrBatch; ++u) {
+              const int i = base + 64 * u;
+              if (i < m) st_agent(col + i, csub(t[u], cmul(vcol[i], tw)));
+            }
+          }
+        }
+        // row j of the updated column: register j / 64 of lane j % 64 (or
+        // memory beyond the registers)
+        double2 rj2 = make_double2(0.0, 0.0);
+        if (j < 64 * kQrCache) {
+#pragma unroll
+          for (int u = 0; u < kQrCache; ++u)
+            if (u == (j >> 6)) rj2 = res[u];
+          rj2.x = __shfl(rj2.x, j & 63, 64);
+          rj2.y = __shfl(rj2.y, j & 63, 64);
+        } else {
+          __threadfence_block();
+          rj2 = ld_agent(col + j);
+        }
+        double nv = vn1[q];
+        if (nv != 0.0) {
+          const double rj = sqrt(cabs2(rj2));
+          double temp = 1.0 - (rj / nv) * (rj / nv);
+          temp = temp > 0.0 ? temp : 0.0;
+          const double ratio = nv / vn2[q];
+          const double temp2 = temp * ratio * ratio;
+          if (temp2 <= tol3z) {
+            // recompute the norm of rows j+1.. from the registers and, beyond
+            // them, from memory (this lane's own stores: complete them before
+            // reading back)
+            __threadfence_block();
+            double pp = 0.0;
+#pragma unroll
+            for (int u = 0; u < kQrCache; ++u) {
+              const int i = lane + 64 * u;
+              if (i > j && i < m) pp += cabs2(res[u]);
+            }
+            for (int i = max(j + 1, 64 * kQrCache) + lane; i < m; i += 64)
+              pp += cabs2(ld_agent(col + i));
+            nv = (j + 1 < m) ? sqrt(wave_sum(pp)) : 0.0;
+            if (lane == 0) { vn1[q] = nv; vn2[q] = nv; }
+          } else {
+            nv = nv * sqrt(temp);
+            if (lane == 0) vn1[q] = nv;
+          }
+        }
+        wsum += nv * nv;
+        better(wmax, wcol, nv, c);
+        continue;
+      }
       // rows j .. j + 64 kQrCache - 1 stay in registers between the dot
       // product and the update; all loads are issued in batches
       double2 cache[kQrCache];
