@@ -162,17 +162,11 @@ __device__ __forceinline__ void st_agent(double2* p, double2 v) {
   st_agent(&p->y, v.y);
 }
 
-// Grid-wide barrier for a cooperatively launched grid (all workgroups
-// co-resident). bar[0] is zeroed before the launch and only grows; the
-// barrier of generation `epoch` waits until epoch * nblocks arrivals. Every
-// wavefront first waits for its outstanding memory operations (s_waitcnt 0),
-// so its agent-scope stores are performed before its workgroup arrives.
-constexpr unsigned kBarrierWords = 64;
-// words of the synchronisation buffer: the grid counter, then (Jacobi) a pair
-// barrier and a pair completion counter per block pair and a version per block
-constexpr unsigned kSyncMaxBlocks = 8192;
-constexpr unsigned kSyncWords = kBarrierWords + 2 * kSyncMaxBlocks;
-
+// Barrier for a group of co-resident workgroups of a cooperative launch.
+// bar[0] is zeroed before the launch and only grows; the barrier of
+// generation `epoch` waits until epoch * nblocks arrivals. Every wavefront
+// first waits for its outstanding memory operations (s_waitcnt 0), so its
+// agent-scope stores are performed before its workgroup arrives.
 __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks,
                                              unsigned& epoch) {
   __builtin_amdgcn_s_waitcnt(0);
@@ -181,6 +175,48 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned nblocks,
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned target = epoch * nblocks;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+}
+
+// Grid-wide barrier over all nblocks workgroups of a launch (or of one
+// problem of a batched launch; bid is the workgroup's index in it), in two
+// levels: workgroup bid arrives on the counter of group bid % kBarrierGroups,
+// the last arrival of a group on the top counter bar[0], and every
+// workgroup waits on the top counter. With one counter, the arrivals of 256
+// workgroups on one word serialise (measured 3.7 us per barrier on MI355X
+// against 2.2 us in two levels with eight groups). The counters are
+// kBarrierStride words apart; all kBarrierWords are zeroed before the
+// launch. Small grids use the single counter.
+constexpr unsigned kBarrierGroups = 8;
+constexpr unsigned kBarrierStride = 64;
+constexpr unsigned kBarrierWords = kBarrierStride * (kBarrierGroups + 1);
+// words of the synchronisation buffer: the grid counters, then (Jacobi) a
+// pair barrier and a pair completion counter per block pair and a version per
+// block
+constexpr unsigned kSyncMaxBlocks = 8192;
+constexpr unsigned kSyncWords = kBarrierWords + 2 * kSyncMaxBlocks;
+
+__device__ __forceinline__ void grid_barrier_all(unsigned* bar, unsigned nblocks,
+                                                 unsigned bid, unsigned& epoch) {
+  if (nblocks < 4 * kBarrierGroups) {
+    grid_barrier(bar, nblocks, epoch);
+    return;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  epoch += 1;
+  if (threadIdx.x == 0) {
+    const unsigned g = bid % kBarrierGroups;
+    const unsigned gsize = nblocks / kBarrierGroups + (g < nblocks % kBarrierGroups ? 1u : 0u);
+    unsigned* grp = bar + kBarrierStride * (g + 1);
+    const unsigned old =
+        __hip_atomic_fetch_add(grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == epoch * gsize - 1)
+      __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned target = epoch * kBarrierGroups;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
       __builtin_amdgcn_s_sleep(1);
   }

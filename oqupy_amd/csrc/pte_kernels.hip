@@ -700,7 +700,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
     __syncthreads();
   }
   if (tid == 0) atomicAdd(a.fro2, red[0]);
-  grid_barrier(a.bar, nblk, epoch);
+  grid_barrier_all(a.bar, nblk, bid, epoch);
   const double fro = sqrt(__hip_atomic_load(a.fro2, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT));
   const double abs_tol = a.abs_tol_scale * fro;
@@ -924,7 +924,7 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
       PHASE(5);
       RTRACE(7);
     }
-    grid_barrier(a.bar, nblk, epoch);
+    grid_barrier_all(a.bar, nblk, bid, epoch);
     if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED,
                           __HIP_MEMORY_SCOPE_AGENT) == 0) {
       converged = true;
@@ -1004,7 +1004,7 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
     __syncthreads();
   }
   if (tid == 0) atomicAdd(a.fro2, red[0]);
-  grid_barrier(a.bar, nblk, epoch);
+  grid_barrier_all(a.bar, nblk, bid, epoch);
   const double fro = sqrt(__hip_atomic_load(a.fro2, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT));
   const double abs_tol = a.abs_tol_scale * fro;
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
       ++gen;
       RTRACE(7);
     }
-    grid_barrier(a.bar, nblk, epoch);
+    grid_barrier_all(a.bar, nblk, bid, epoch);
     if (__hip_atomic_load(&a.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
       converged = true;
       break;
@@ -1551,7 +1551,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
     better(wmax, wcol, nrm, c);
   }
   qrcp_post(a.cand + blk, wmax, wcol, wsum, cw_max, cw_col, cw_sum);
-  grid_barrier(a.bar, nblk, epoch);
+  grid_barrier_all(a.bar, nblk, blk, epoch);
   double total;
   int p;
   qrcp_reduce(a.cand, nblk, slot, slotv, slotc, parity, &p, &total);
@@ -1814,7 +1814,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
     qrcp_post(a.cand + ((j + 1) & 1) * nblk + blk, wmax, wcol, wsum, cw_max, cw_col,
               cw_sum);
     PHASE(10);
-    grid_barrier(a.bar, nblk, epoch);
+    grid_barrier_all(a.bar, nblk, blk, epoch);
     PHASE(11);
     qrcp_reduce(a.cand + ((j + 1) & 1) * nblk, nblk, slot, slotv, slotc, parity, &p,
                 &total);
