@@ -162,6 +162,32 @@ __device__ __forceinline__ void st_agent(double2* p, double2 v) {
   st_agent(&p->y, v.y);
 }
 
+// The same agent-scope accesses for a whole complex entry in one 16-byte
+// buffer instruction (the pair of 8-byte atomics above takes two). The
+// buffer resource covers `bytes` bytes from `base` (accesses beyond it load
+// zero and store nothing); the cache policy is sc1, the bits the compiler
+// gives the agent-scope atomics on gfx950.
+constexpr int kBufAgentPolicy = 16;                  // sc1
+constexpr int kBufResourceWord3 = 0x00020000;        // raw 32-bit buffer (gfx9)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t agent_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes,
+                                           kBufResourceWord3);
+}
+typedef int int4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double2 ld_agent16(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+  const int4v d = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, kBufAgentPolicy);
+  double2 v;
+  __builtin_memcpy(&v, &d, sizeof(v));
+  return v;
+}
+__device__ __forceinline__ void st_agent16(__amdgpu_buffer_rsrc_t r, unsigned byte_off,
+                                           double2 v) {
+  int4v d;
+  __builtin_memcpy(&d, &v, sizeof(v));
+  __builtin_amdgcn_raw_buffer_store_b128(d, r, (int)byte_off, 0, kBufAgentPolicy);
+}
+
 // Barrier for a group of co-resident workgroups of a cooperative launch.
 // bar[0] is zeroed before the launch and only grows; the barrier of
 // generation `epoch` waits until epoch * nblocks arrivals. Every wavefront

@@ -293,7 +293,7 @@ struct Workspace {
   // PTE_JACOBI_TILE=0; slices keep at least tile_rows rows of X
   // (PTE_JACOBI_TILE_ROWS); a slice's X and V rows together fit kTileMaxRows
   bool tile_enabled = true;
-  int tile_rows = 32;
+  int tile_rows = 64;
   int tile_coop_blocks = 0;
   bool small_enabled = true; // tuning: PTE_SMALL_SVD=0 sends every SVD to the grid path
   int max_coop_blocks = 0;

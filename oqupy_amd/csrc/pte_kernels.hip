@@ -980,6 +980,15 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
   const int xr = a.xrows_per, vr = a.vrows_per, tr = xr + vr;
   const int x_lo = min(a.m, slice * xr), x_hi = min(a.m, x_lo + xr);
   const int v_lo = min(a.n, slice * vr), v_hi = min(a.n, v_lo + vr);
+  // 16-byte agent-coherent accesses to X, V and the partial Grams (every
+  // offset of the tile path fits 31 bits: at most 16 slices of 192 rows and
+  // 8192 columns)
+  const __amdgpu_buffer_rsrc_t xrs =
+      agent_rsrc(a.X, (unsigned)min((long)a.ldx * a.n * 16, 0x7fffffffL));
+  const __amdgpu_buffer_rsrc_t vrs =
+      agent_rsrc(a.V, (unsigned)min((long)a.ldv * a.n * 16, 0x7fffffffL));
+  const __amdgpu_buffer_rsrc_t grs =
+      agent_rsrc(a.gpart, (unsigned)min(2L * (a.nb / 2) * a.slices * (JP * JP) * 16, 0x7fffffffL));
   unsigned epoch = 0;
   for (int e = tid; e < (JP - 1) * JB; e += JT) {
     int p, q;
@@ -1059,9 +1068,10 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
             v[q] = make_double2(0.0, 0.0);
             if (e < ntile && cols[col] >= 0) {
               if (row < xr) {
-                if (x_lo + row < x_hi) v[q] = ld_agent(a.X + (x_lo + row + (long)cols[col] * a.ldx));
+                if (x_lo + row < x_hi)
+                  v[q] = ld_agent16(xrs, (unsigned)(x_lo + row + (long)cols[col] * a.ldx) * 16u);
               } else if (v_lo + row - xr < v_hi) {
-                v[q] = ld_agent(a.V + (v_lo + row - xr + (long)cols[col] * a.ldv));
+                v[q] = ld_agent16(vrs, (unsigned)(v_lo + row - xr + (long)cols[col] * a.ldv) * 16u);
               }
             }
           }
@@ -1093,7 +1103,7 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
             const int gi = 16 * I + 4 * r + lk, gj = 16 * J + lr;
             const double2 g = make_double2(p1[r] - p2[r], p3[r] - p1[r] - p2[r]);
             if (R == 1) Gs[0][gi][gj] = g;
-            else st_agent(mine + (gi * JP + gj), g);
+            else st_agent16(grs, (unsigned)((mine - a.gpart) + gi * JP + gj) * 16u, g);
           }
         }
         if (R > 1) {
@@ -1101,8 +1111,9 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
           grid_barrier(pair_bar + pair, R, pair_epoch);
           RTRACE(3);
           // identical summation order (slice 0, 1, ...) in every workgroup
-          // of the pair; loads of eight slices in flight together
-          constexpr int SB = 8;
+          // of the pair; the loads of all slices (at most 16) in flight
+          // together
+          constexpr int SB = 16;
           double2 acc[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
@@ -1113,8 +1124,10 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
             for (int t = 0; t < SB; ++t)
 #pragma unroll
               for (int q = 0; q < 4; ++q)
-                v[t][q] = (sl0 + t < R) ? ld_agent(gp + (long)(sl0 + t) * (JP * JP) + tid + JT * q)
-                                        : make_double2(0.0, 0.0);
+                v[t][q] = (sl0 + t < R)
+                              ? ld_agent16(grs, (unsigned)((gp - a.gpart) + (long)(sl0 + t) * (JP * JP) +
+                                                           tid + JT * q) * 16u)
+                              : make_double2(0.0, 0.0);
 #pragma unroll
             for (int t = 0; t < SB; ++t)
 #pragma unroll
@@ -1208,9 +1221,10 @@ __device__ __forceinline__ void jacobi_tile_body(const JacobiArgs& a, const int 
             if (gc < 0) continue;
             const double2 val = tile[row * kTileLd + col];
             if (row < xr) {
-              if (x_lo + row < x_hi) st_agent(a.X + (x_lo + row + (long)gc * a.ldx), val);
+              if (x_lo + row < x_hi)
+                st_agent16(xrs, (unsigned)(x_lo + row + (long)gc * a.ldx) * 16u, val);
             } else if (v_lo + row - xr < v_hi) {
-              st_agent(a.V + (v_lo + row - xr + (long)gc * a.ldv), val);
+              st_agent16(vrs, (unsigned)(v_lo + row - xr + (long)gc * a.ldv) * 16u, val);
             }
           }
           if (tid == 0) st_agent(a.flags + sweep, 1);
@@ -1517,10 +1531,11 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
   for (int u = 0; u < kQrCache; ++u) res[u] = make_double2(0.0, 0.0);
   if (wave < nloc && !(j0 > 0 && ld_agent(a.done + blk + wave * nblk))) {
     const double2* col = a.A + (long)(blk + wave * nblk) * lda;
+    const __amdgpu_buffer_rsrc_t crs = agent_rsrc(col, (unsigned)m * 16u);
 #pragma unroll
     for (int u = 0; u < kQrCache; ++u) {
       const int i = lane + 64 * u;
-      if (i >= j0 && i < m) res[u] = ld_agent(col + i);
+      if (i >= j0 && i < m) res[u] = ld_agent16(crs, (unsigned)(i) * 16u);
     }
     resident = true;
   }
@@ -1531,13 +1546,14 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       continue;
     }
     const double2* col = a.A + (long)c * lda;
+    const __amdgpu_buffer_rsrc_t crs = agent_rsrc(col, (unsigned)m * 16u);
     double part = 0.0;
     for (int base = j0 + lane; base < m; base += 64 * kQrBatch) {
       double2 t[kQrBatch];
 #pragma unroll
       for (int u = 0; u < kQrBatch; ++u) {
         const int i = base + 64 * u;
-        t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+        t[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int u = 0; u < kQrBatch; ++u) part += cabs2(t[u]);
@@ -1575,13 +1591,14 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
     // column is read once into vcol (loads issued in batches, so that their
     // latencies overlap), then scaled in LDS
     const double2* pc = a.A + (long)p * lda;
+    const __amdgpu_buffer_rsrc_t prs = agent_rsrc(pc, (unsigned)m * 16u);
     double part = 0.0;
     for (int base = j + tid; base < m; base += 256 * kQrBatch) {
       double2 t[kQrBatch];
 #pragma unroll
       for (int q = 0; q < kQrBatch; ++q) {
         const int i = base + 256 * q;
-        t[q] = (i < m) ? ld_agent(pc + i) : make_double2(0.0, 0.0);
+        t[q] = (i < m) ? ld_agent16(prs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int q = 0; q < kQrBatch; ++q) {
@@ -1631,6 +1648,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       if (done[q]) continue;
       const int c = blk + q * nblk;
       double2* col = a.A + (long)c * lda;
+      const __amdgpu_buffer_rsrc_t crs = agent_rsrc(col, (unsigned)m * 16u);
       if (q == wave && resident) {
         // the resident column: rows below 64 kQrCache from registers, the
         // rest (if any) streamed
@@ -1647,7 +1665,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
 #pragma unroll
             for (int u = 0; u < kQrBatch; ++u) {
               const int i = base + 64 * u;
-              t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+              t[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < kQrBatch; ++u) {
@@ -1663,7 +1681,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
             const int i = lane + 64 * u;
             if (i >= j && i < m) {
               res[u] = csub(res[u], cmul(vcol[i], tw));
-              st_agent(col + i, res[u]);
+              st_agent16(crs, (unsigned)(i) * 16u, res[u]);
             }
           }
           for (int base = beyond + lane; base < m; base += 64 * kQrBatch) {
@@ -1671,12 +1689,12 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
 #pragma unroll
             for (int u = 0; u < kQrBatch; ++u) {
               const int i = base + 64 * u;
-              t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+              t[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < kQrBatch; ++u) {
               const int i = base + 64 * u;
-              if (i < m) st_agent(col + i, csub(t[u], cmul(vcol[i], tw)));
+              if (i < m) st_agent16(crs, (unsigned)(i) * 16u, csub(t[u], cmul(vcol[i], tw)));
             }
           }
         }
@@ -1691,7 +1709,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
           rj2.y = __shfl(rj2.y, j & 63, 64);
         } else {
           __threadfence_block();
-          rj2 = ld_agent(col + j);
+          rj2 = ld_agent16(crs, (unsigned)(j) * 16u);
         }
         double nv = vn1[q];
         if (nv != 0.0) {
@@ -1712,7 +1730,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
               if (i > j && i < m) pp += cabs2(res[u]);
             }
             for (int i = max(j + 1, 64 * kQrCache) + lane; i < m; i += 64)
-              pp += cabs2(ld_agent(col + i));
+              pp += cabs2(ld_agent16(crs, (unsigned)(i) * 16u));
             nv = (j + 1 < m) ? sqrt(wave_sum(pp)) : 0.0;
             if (lane == 0) { vn1[q] = nv; vn2[q] = nv; }
           } else {
@@ -1730,7 +1748,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
 #pragma unroll
       for (int u = 0; u < kQrCache; ++u) {
         const int i = j + lane + 64 * u;
-        cache[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+        cache[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
       }
       if (reflect) {
         double2 w = make_double2(0.0, 0.0);
@@ -1744,7 +1762,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
 #pragma unroll
           for (int u = 0; u < kQrBatch; ++u) {
             const int i = base + 64 * u;
-            t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+            t[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
           }
 #pragma unroll
           for (int u = 0; u < kQrBatch; ++u) {
@@ -1760,7 +1778,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
           const int i = j + lane + 64 * u;
           if (i < m) {
             cache[u] = csub(cache[u], cmul(vcol[i], tw));
-            st_agent(col + i, cache[u]);
+            st_agent16(crs, (unsigned)(i) * 16u, cache[u]);
           }
         }
         for (int base = j + lane + 64 * kQrCache; base < m; base += 64 * kQrBatch) {
@@ -1768,12 +1786,12 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
 #pragma unroll
           for (int u = 0; u < kQrBatch; ++u) {
             const int i = base + 64 * u;
-            t[u] = (i < m) ? ld_agent(col + i) : make_double2(0.0, 0.0);
+            t[u] = (i < m) ? ld_agent16(crs, (unsigned)(i) * 16u) : make_double2(0.0, 0.0);
           }
 #pragma unroll
           for (int u = 0; u < kQrBatch; ++u) {
             const int i = base + 64 * u;
-            if (i < m) st_agent(col + i, csub(t[u], cmul(vcol[i], tw)));
+            if (i < m) st_agent16(crs, (unsigned)(i) * 16u, csub(t[u], cmul(vcol[i], tw)));
           }
         }
       }
@@ -1799,7 +1817,7 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
             const int i = j + lane + 64 * u;
             if (i > j && i < m) pp += cabs2(cache[u]);
           }
-          for (int i = j + lane + 64 * kQrCache; i < m; i += 64) pp += cabs2(ld_agent(col + i));
+          for (int i = j + lane + 64 * kQrCache; i < m; i += 64) pp += cabs2(ld_agent16(crs, (unsigned)(i) * 16u));
           nv = (j + 1 < m) ? sqrt(wave_sum(pp)) : 0.0;
           if (lane == 0) { vn1[q] = nv; vn2[q] = nv; }
         } else {
