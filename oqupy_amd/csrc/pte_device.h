@@ -169,6 +169,7 @@ __device__ __forceinline__ void st_agent(double2* p, double2 v) {
 // gives the agent-scope atomics on gfx950.
 constexpr int kBufAgentPolicy = 16;                  // sc1
 constexpr int kBufResourceWord3 = 0x00020000;        // raw 32-bit buffer (gfx9)
+constexpr long kBufMaxEntries = 1L << 27;            // complex entries in 2^31 bytes
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t agent_rsrc(const void* base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes,

@@ -637,13 +637,18 @@ constexpr int PF = 4;
 
 __device__ __forceinline__ void load_chunk(double2 (&out)[4], const double2* base, long ld,
                                            int r0, int hi, const int* cols) {
+  // 16-byte agent-coherent loads within the first 2^31 bytes of the
+  // matrix (the buffer offset range), the pair of 8-byte loads beyond
+  const __amdgpu_buffer_rsrc_t rs = agent_rsrc(base, 0x7fffffffu);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int e = threadIdx.x + JT * q;
     const int rr = e % JR, cc = e / JR;
     const int gr = r0 + rr, gc = cols[cc];
-    out[q] = (gr < hi && gc >= 0) ? ld_agent(base + (gr + (long)gc * ld))
-                                  : make_double2(0.0, 0.0);
+    const long off = gr + (long)gc * ld;
+    out[q] = (gr < hi && gc >= 0)
+                 ? (off < kBufMaxEntries ? ld_agent16(rs, (unsigned)off * 16u) : ld_agent(base + off))
+                 : make_double2(0.0, 0.0);
   }
 }
 
@@ -899,7 +904,13 @@ __device__ __forceinline__ void jacobi_body(const JacobiArgs& a, const int bid,
               const int e = tid + JT * q;
               const int row = e % JR, col = e / JR;
               const int gr = r0 + row, gc = cols[col];
-              if (gr < hi && gc >= 0) st_agent(base + (gr + (long)gc * ld), Xs[row][col]);
+              const long off = gr + (long)gc * ld;
+              if (gr < hi && gc >= 0) {
+                if (off < kBufMaxEntries)
+                  st_agent16(agent_rsrc(base, 0x7fffffffu), (unsigned)off * 16u, Xs[row][col]);
+                else
+                  st_agent(base + off, Xs[row][col]);
+              }
             }
             __syncthreads();
           }
