@@ -2256,42 +2256,78 @@ small_svd_batch_kernel(const SmallSvdArgs* probs) {
   small_svd_body(a);
 }
 
-// Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR: one
-// workgroup per column, the column held in LDS, reflectors applied from
-// the last to the first; row i = a*B + b goes to dst[j*dj + a*da + b*db].
+// Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR, the
+// reflectors applied from the last to the first to a column held in LDS;
+// row i = a*B + b goes to dst[j*dj + a*da + b*db].
+// One column per workgroup, the reflector entries of each step loaded into
+// registers in one batch (rows t + tid + 256 u, u < kApplyQRegs; rows beyond
+// on demand) and those of the next step issued before this step's
+// reduction, so that their latency overlaps the work.
+constexpr int kApplyQRegs = 8;
+
 __global__ void __launch_bounds__(256)
 apply_q_kernel(ApplyQArgs a) {
   extern __shared__ double2 cvec[];
   __shared__ double slot[2][4], sloti[2][4];
   int parity = 0;
+  const int tid = threadIdx.x;
   const int k = a.info->rank;
   const int r = *a.qrank;
+  const int m = a.m;
   const long dj = a.dj0 + a.djk * (long)k;
   const long da = a.da0 + a.dak * (long)k;
   const long db = a.db0 + a.dbk * (long)k;
   for (int j = blockIdx.x; j < k; j += gridDim.x) {
     const double2* w = a.W + (long)a.perm[j] * a.ldw;
-    for (int i = threadIdx.x; i < a.m; i += blockDim.x)
+    for (int i = tid; i < m; i += blockDim.x)
       cvec[i] = (i < r) ? w[i] : make_double2(0.0, 0.0);
     __syncthreads();
-    for (int t = r - 1; t >= 0; --t) {
-      const double2 tau = a.tau[t];
-      if (tau.x == 0.0 && tau.y == 0.0) continue;
+    // reflector t's entries (unscaled) for rows t + tid + 256 u
+    double2 vn[kApplyQRegs];
+    auto fetch = [&](int t) {
       const double2* v = a.A + (long)a.piv[t] * a.lda;
+#pragma unroll
+      for (int u = 0; u < kApplyQRegs; ++u) {
+        const int i = t + tid + 256 * u;
+        vn[u] = (i < m) ? v[i] : make_double2(0.0, 0.0);
+      }
+    };
+    if (r > 0) fetch(r - 1);
+    for (int t = r - 1; t >= 0; --t) {
+      double2 vc[kApplyQRegs];
+#pragma unroll
+      for (int u = 0; u < kApplyQRegs; ++u) vc[u] = vn[u];
+      const double2 tau = a.tau[t];
       const double2 sc = a.vscale[t];
+      const double2* v = a.A + (long)a.piv[t] * a.lda;
+      if (t > 0) fetch(t - 1);
+      if (tau.x == 0.0 && tau.y == 0.0) continue;
       double2 acc = make_double2(0.0, 0.0);
-      for (int i = t + threadIdx.x; i < a.m; i += blockDim.x) {
-        const double2 vi = (i == t) ? make_double2(1.0, 0.0) : cmul(v[i], sc);
+#pragma unroll
+      for (int u = 0; u < kApplyQRegs; ++u) {
+        const int i = t + tid + 256 * u;
+        if (i < m) {
+          vc[u] = (i == t) ? make_double2(1.0, 0.0) : cmul(vc[u], sc);
+          cfmac(acc, vc[u], cvec[i]);
+        }
+      }
+      for (int i = t + tid + 256 * kApplyQRegs; i < m; i += blockDim.x) {
+        const double2 vi = cmul(v[i], sc);
         cfmac(acc, vi, cvec[i]);
       }
       const double2 tw = cmul(tau, block_sum256c(acc, slot, sloti, parity));
-      for (int i = t + threadIdx.x; i < a.m; i += blockDim.x) {
-        const double2 vi = (i == t) ? make_double2(1.0, 0.0) : cmul(v[i], sc);
+#pragma unroll
+      for (int u = 0; u < kApplyQRegs; ++u) {
+        const int i = t + tid + 256 * u;
+        if (i < m) cvec[i] = csub(cvec[i], cmul(vc[u], tw));
+      }
+      for (int i = t + tid + 256 * kApplyQRegs; i < m; i += blockDim.x) {
+        const double2 vi = cmul(v[i], sc);
         cvec[i] = csub(cvec[i], cmul(vi, tw));
       }
       __syncthreads();
     }
-    for (int i = threadIdx.x; i < a.m; i += blockDim.x) {
+    for (int i = tid; i < m; i += blockDim.x) {
       const int aa = i / a.B, b = i % a.B;
       a.dst[(long)j * dj + (long)aa * da + (long)b * db] = cvec[i];
     }
