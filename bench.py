@@ -42,8 +42,12 @@ PHASE_STEPS = {"c2": 40, "c3": 0, "c5": 0}
 # FETCH_SIZE and WRITE_SIZE runs of the same command and window, scaled by
 # what the counters read for a plain copy (keyed by config, precision, rank
 # form, phase steps, warm-up, timed steps)
-TRAFFIC = {}
-TRAFFIC_SOURCE = None
+TRAFFIC = {
+    # FETCH_SIZE 1330623.2 KiB x 2.000 + WRITE_SIZE 1077194.8 KiB x 1.000 per
+    # Jacobi dispatch, mean over the window's 6760 dispatches
+    ("c2", "c128", "exact", 40, 5, 20): 3828163738,
+}
+TRAFFIC_SOURCE = "profiles/hbm_traffic_svd_kernels_pt_tempo_dkmax200_steps47-66.md"
 
 # PT-TEMPO workloads: headline spin-boson (c2), two qubits with a collective
 # bath (c3, Liouville dimension 16), and the long-memory, tight-tolerance run
@@ -139,7 +143,7 @@ def svd_flops(m, n):
 
 
 def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None,
-            export_state=False, rank_theta=None):
+            export_state=False, rank_theta=None, progress=False):
     """The engine on the workload: initialise, `warmup` untimed steps, then
     `steps` timed steps. Returns a dict: elapsed seconds, Jacobi kernel
     statistics of the window, per-step statistics, the SVD log of the window
@@ -176,8 +180,13 @@ def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None,
     backend.svd_log(200_000_000)
     before = backend.svd_kernel_stats()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         backend.compute_step()
+        if progress:
+            # a line per timed step for long profiled runs (a few microseconds
+            # against a step of about two seconds)
+            print(f"[bench] timed step {i + 1}/{steps}", file=sys.stderr,
+                  flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     after = backend.svd_kernel_stats()
@@ -580,6 +589,18 @@ def trace_summary(directory, kernel, last):
         out["avg_ms_all"] = sum(dur) / len(dur)
         out["window_dispatches"] = len(win)
         out["avg_ms_window"] = sum(win) / len(win)
+        # GPU busy time from the window's first dispatch of this kernel to
+        # the last kernel of the run (union of all kernels' intervals)
+        t_start = int(trace[-len(win)]["Start_Timestamp"])
+        spans = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                       for r in rows("*kernel_trace.csv")
+                       if int(r["Start_Timestamp"]) >= t_start)
+        busy, last_end = 0, t_start
+        for s0, e0 in spans:
+            busy += max(0, e0 - max(s0, last_end))
+            last_end = max(last_end, e0)
+        out["window_wall_ms"] = (last_end - t_start) / 1e6
+        out["window_busy_frac"] = busy / max(last_end - t_start, 1)
     counters = [r for r in rows("*counter_collection.csv")
                 if kernel in r.get("Kernel_Name", "")]
     if counters:
@@ -612,6 +633,8 @@ def main():
                     help="BLAS threads of the CPU baseline (default: the "
                          "host's cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--progress", action="store_true",
+                    help="print a line after every timed step (profiled runs)")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the whole-run TEMPO accuracy check")
     ap.add_argument("--config", choices=("c2", "c1", "c3", "c5"), default="c2",
@@ -645,7 +668,7 @@ def main():
     ap.add_argument("--trace-summary", metavar="DIR", default=None,
                     help="summarise a rocprofv3 output directory for --kernel "
                          "over its last --last dispatches and exit")
-    ap.add_argument("--kernel", default="jacobi_kernel")
+    ap.add_argument("--kernel", default="pte::jacobi")
     ap.add_argument("--last", type=int, default=0)
     args = ap.parse_args()
     if args.trace_summary:
@@ -695,7 +718,8 @@ def main():
     elif args.rank != "exact":
         predictor = args.rank
     run = gpu_run(warm, args.steps, table, args.precision, predictor,
-                  export_state=not args.no_cpu, rank_theta=args.rank_theta)
+                  export_state=not args.no_cpu, rank_theta=args.rank_theta,
+                  progress=args.progress)
     elapsed, svd, window, log = run["elapsed"], run["svd"], run["window"], \
         run["log"]
     value = args.steps / elapsed
@@ -707,7 +731,8 @@ def main():
     svd_avg_ms = svd["ms"] / max(svd["count"], 1)
     achieved = svd["flops"] / max(svd["ms"], 1e-12) / 1e9   # TFLOP/s
     roofline = {
-        "kernel": "jacobi_kernel (blocked one-sided Jacobi SVD, FP64)",
+        "kernel": "pte::jacobi (jacobi_tile_kernel and jacobi_kernel: blocked "
+                  "one-sided Jacobi SVD, FP64)",
         "bound": "mfma", "achieved": round(achieved, 4),
         "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP64_MFMA_PEAK_TFLOPS, 6),
