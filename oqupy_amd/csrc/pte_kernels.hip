@@ -1662,14 +1662,20 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
       const __amdgpu_buffer_rsrc_t crs = agent_rsrc(col, (unsigned)m * 16u);
       if (q == wave && resident) {
         // the resident column: rows below 64 kQrCache from registers, the
-        // rest (if any) streamed
+        // rest (if any) streamed. The register rows are processed without
+        // branches: rows outside [j, m) take a zero reflector entry (their
+        // update is then exact: x - 0 = x); only the stores are predicated,
+        // after all the arithmetic.
         const int beyond = max(j, 64 * kQrCache);
         if (reflect) {
+          double2 vr[kQrCache];
           double2 w = make_double2(0.0, 0.0);
 #pragma unroll
           for (int u = 0; u < kQrCache; ++u) {
             const int i = lane + 64 * u;
-            if (i >= j && i < m) cfmac(w, vcol[i], res[u]);
+            const double2 vi = vcol[min(i, m - 1)];
+            vr[u] = (i >= j && i < m) ? vi : make_double2(0.0, 0.0);
+            cfmac(w, vr[u], res[u]);
           }
           for (int base = beyond + lane; base < m; base += 64 * kQrBatch) {
             double2 t[kQrBatch];
@@ -1688,12 +1694,11 @@ __device__ __forceinline__ void qrcp_body(const QrcpArgs& a, const int blk, cons
           w.y = wave_sum(w.y);
           const double2 tw = cmul(tc, w);
 #pragma unroll
+          for (int u = 0; u < kQrCache; ++u) res[u] = csub(res[u], cmul(vr[u], tw));
+#pragma unroll
           for (int u = 0; u < kQrCache; ++u) {
             const int i = lane + 64 * u;
-            if (i >= j && i < m) {
-              res[u] = csub(res[u], cmul(vcol[i], tw));
-              st_agent16(crs, (unsigned)(i) * 16u, res[u]);
-            }
+            if (i >= j && i < m) st_agent16(crs, (unsigned)(i) * 16u, res[u]);
           }
           for (int base = beyond + lane; base < m; base += 64 * kQrBatch) {
             double2 t[kQrBatch];
