@@ -420,13 +420,39 @@ def batch_main(args):
         "roofline": None, "cpu_baseline": None}))
 
 
-def tempo_c1(precision="c128", cpu=True):
+class _OtherDriver:
+    """While active, the oracle's truncated SVD uses LAPACK's gesvd instead
+    of gesdd: the spread between two backward-stable drivers is how far the
+    FP64 reference itself defines a result whose truncations sit at
+    near-ties (loose epsrel)."""
+
+    def __enter__(self):
+        import scipy.linalg as sl
+        from oracle import chain
+        self._chain, self._orig = chain, chain.truncated_svd
+
+        def gesvd(mat, eps, spectra=None):
+            u, s, vh = sl.svd(mat, full_matrices=False, lapack_driver="gesvd")
+            if spectra is not None:
+                spectra.append((mat.shape[0], mat.shape[1], s.copy()))
+            k = chain.truncation_rank(s, eps)
+            return u[:, :k], s[:k], vh[:k]
+        chain.truncated_svd = gesvd
+        return self
+
+    def __exit__(self, *exc):
+        self._chain.truncated_svd = self._orig
+        return False
+
+
+def tempo_c1(precision="c128", cpu=True, spread=False):
     """TEMPO on the short-memory spin-boson workload (H_S = sigma_x / 2,
     coupling sigma_z / 2, Ohmic alpha 0.3, w_c 3, T 0, rho_0 = |up><up|,
     dt 0.05, dkmax 40, epsrel 1e-6, 200 steps) on the engine and, with
     `cpu`, on the FP64 oracle with the same influences. Returns (GPU
     seconds for steps 2..200, oracle seconds for the same steps or None,
-    max |d<sigma_z>| over all 201 times or None)."""
+    max |d<sigma_z>| over all 201 times or None) and, with `spread`, the
+    oracle's own spread between two LAPACK drivers."""
     import numpy as np
     import torch
     import oqupy_amd as oqupy
@@ -457,28 +483,38 @@ def tempo_c1(precision="c128", cpu=True):
         return elapsed, None, None
     props = system.get_propagators(0.05, 0.0, params.subdiv_limit,
                                    params.liouvillian_epsrel)
-    ref = TempoOracle(rho0, lambda dk: table.get(dk), bath.unitary_transform,
-                      props, np.ones(4), np.ones(4), dkmax, 1e-6)
-    _, st = ref.initialize()
-    states = [st.reshape(2, 2)]
-    _, st = ref.compute_step()
-    states.append(st.reshape(2, 2))
-    t0 = time.perf_counter()
-    for _ in range(n - 1):
+    sz = op.sigma("z")
+
+    def oracle_run():
+        ref = TempoOracle(rho0, lambda dk: table.get(dk), bath.unitary_transform,
+                          props, np.ones(4), np.ones(4), dkmax, 1e-6)
+        _, st = ref.initialize()
+        states = [st.reshape(2, 2)]
         _, st = ref.compute_step()
         states.append(st.reshape(2, 2))
-    cpu_elapsed = time.perf_counter() - t0
-    sz = op.sigma("z")
+        t0 = time.perf_counter()
+        for _ in range(n - 1):
+            _, st = ref.compute_step()
+            states.append(st.reshape(2, 2))
+        return (time.perf_counter() - t0,
+                np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
+    cpu_elapsed, sz_ref = oracle_run()
     dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
-                 - np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
-    return elapsed, cpu_elapsed, float(dsz.max())
+                 - sz_ref)
+    if not spread:
+        return elapsed, cpu_elapsed, float(dsz.max())
+    with _OtherDriver():
+        _, sz_alt = oracle_run()
+    return elapsed, cpu_elapsed, float(dsz.max()), float(np.abs(sz_alt - sz_ref).max())
 
 
 def pt_accuracy(precision="c128", rank_predictor=None, rank_theta=None):
     """max |d<sigma_z>| of a whole PT-TEMPO + compute_dynamics run in the
     given form against the FP64 oracle fed the same influences: the
     short-memory spin-boson bath (alpha 0.3, w_c 3, T 0), dt 0.05, dkmax 20,
-    epsrel 1e-6, 80 steps (the oracle finishes it in seconds)."""
+    epsrel 1e-6, 80 steps (the oracle finishes it in seconds). Returns the
+    deviation, the oracle's largest bond and the oracle's spread between
+    two LAPACK drivers."""
     import numpy as np
     import oqupy_amd as oqupy
     from oqupy_amd import operators as op
@@ -507,28 +543,39 @@ def pt_accuracy(precision="c128", rank_predictor=None, rank_theta=None):
     pt = ptt.get_process_tensor()
     dyn = oqupy.compute_dynamics(system=system, process_tensor=pt,
                                  initial_state=rho0)
-    ref = PtTempoOracle(2, lambda dk: table.get(dk), np.ones(4), n, dkmax, eps)
-    ref.compute()
-    tens = ref.mpo_tensors()
-    props = system.get_propagators(0.05, 0.0, None, None)
-    states = compute_dynamics(props, rho0, n, [tens], [compute_caps(tens, 2)])
     sz = op.sigma("z")
+    props = system.get_propagators(0.05, 0.0, None, None)
+
+    def oracle_run():
+        ref = PtTempoOracle(2, lambda dk: table.get(dk), np.ones(4), n, dkmax, eps)
+        ref.compute()
+        tens = ref.mpo_tensors()
+        states = compute_dynamics(props, rho0, n, [tens], [compute_caps(tens, 2)])
+        return (np.real(np.einsum("ij,tji->t", sz, np.asarray(states))),
+                max(ref.bond_dimensions()))
+    sz_ref, bond = oracle_run()
     dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
-                 - np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
-    return float(dsz.max()), max(ref.bond_dimensions())
+                 - sz_ref)
+    with _OtherDriver():
+        sz_alt, _ = oracle_run()
+    return float(dsz.max()), bond, float(np.abs(sz_alt - sz_ref).max())
 
 
 def accuracy(precision="c128", rank_predictor=None, rank_theta=None):
     """The accuracy half of the metric on whole runs that the oracle
     finishes in seconds: max |d<sigma_z>| against the FP64 oracle of the
     200-step TEMPO workload (plain form) and of a 80-step PT-TEMPO +
-    compute_dynamics run in the benchmarked form."""
-    elapsed, cpu_elapsed, dsz = tempo_c1()
-    pt_dsz, pt_bond = pt_accuracy(precision, rank_predictor, rank_theta)
+    compute_dynamics run in the benchmarked form, each beside the oracle's
+    own spread between two LAPACK drivers (at epsrel 1e-6 truncations at
+    near-ties make both runs differ from the oracle by more than rounding:
+    the drivers' spread is the scale against which to read the deviation)."""
+    elapsed, cpu_elapsed, dsz, spread = tempo_c1(spread=True)
+    pt_dsz, pt_bond, pt_spread = pt_accuracy(precision, rank_predictor, rank_theta)
     return {"tempo": {"workload": "spin-boson TEMPO alpha=0.3 wc=3.0 T=0 "
                                   "dt=0.05 dkmax=40 epsrel=1e-6 N=200 (whole "
                                   "run)",
                       "max_abs_dsigma_z": dsz,
+                      "oracle_lapack_driver_spread": spread,
                       "gpu_steps_per_s": round(199 / elapsed, 3),
                       "oracle_steps_per_s": round(199 / cpu_elapsed, 3)},
             "pt_tempo": {"workload": "spin-boson PT-TEMPO alpha=0.3 wc=3.0 "
@@ -536,6 +583,7 @@ def accuracy(precision="c128", rank_predictor=None, rank_theta=None):
                                      "+ compute_dynamics (whole run), "
                                      "benchmarked form",
                          "max_abs_dsigma_z": pt_dsz,
+                         "oracle_lapack_driver_spread": pt_spread,
                          "max_bond": pt_bond}}
 
 
