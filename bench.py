@@ -189,6 +189,7 @@ def gpu_run(warmup, steps, table, precision="c128", rank_predictor=None,
                   flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    _progress(f"timed window: {steps} steps in {elapsed:.1f} s")
     after = backend.svd_kernel_stats()
     window = backend.step_stats[-steps:]
     svd = {k: after[k] - before[k] for k in after}
@@ -420,6 +421,12 @@ def batch_main(args):
         "roofline": None, "cpu_baseline": None}))
 
 
+def _progress(msg):
+    """A progress line on stderr between the benchmark's phases (never
+    inside the timed window)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 class _OtherDriver:
     """While active, the oracle's truncated SVD uses LAPACK's gesvd instead
     of gesdd: the spread between two backward-stable drivers is how far the
@@ -498,11 +505,13 @@ def tempo_c1(precision="c128", cpu=True, spread=False):
             states.append(st.reshape(2, 2))
         return (time.perf_counter() - t0,
                 np.real(np.einsum("ij,tji->t", sz, np.asarray(states))))
+    _progress("TEMPO oracle")
     cpu_elapsed, sz_ref = oracle_run()
     dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
                  - sz_ref)
     if not spread:
         return elapsed, cpu_elapsed, float(dsz.max())
+    _progress("TEMPO oracle, second LAPACK driver")
     with _OtherDriver():
         _, sz_alt = oracle_run()
     return elapsed, cpu_elapsed, float(dsz.max()), float(np.abs(sz_alt - sz_ref).max())
@@ -553,9 +562,11 @@ def pt_accuracy(precision="c128", rank_predictor=None, rank_theta=None):
         states = compute_dynamics(props, rho0, n, [tens], [compute_caps(tens, 2)])
         return (np.real(np.einsum("ij,tji->t", sz, np.asarray(states))),
                 max(ref.bond_dimensions()))
+    _progress("PT-TEMPO oracle")
     sz_ref, bond = oracle_run()
     dsz = np.abs(np.real(np.einsum("ij,tji->t", sz, np.asarray(dyn.states)))
                  - sz_ref)
+    _progress("PT-TEMPO oracle, second LAPACK driver")
     with _OtherDriver():
         sz_alt, _ = oracle_run()
     return float(dsz.max()), bond, float(np.abs(sz_alt - sz_ref).max())
@@ -799,6 +810,7 @@ def main():
         legs = []
         for threads, budget in ((cores, args.cpu_budget),
                                 (1, args.cpu_budget / 2)):
+            _progress(f"cpu baseline, {threads} BLAS threads")
             rate, n_done, share = cpu_baseline(run["state"], table,
                                                step_flops[0], budget, threads)
             legs.append({"value": round(rate, 6), "cores": threads,
@@ -855,6 +867,7 @@ def main():
         out["config"]["rank_stats_whole_run"] = run["rank"]
         out["config"]["rank_predictor_training"] = training
     if args.config == "c2" and not args.no_accuracy:
+        _progress("accuracy runs")
         out["accuracy"] = accuracy(args.precision, predictor, args.rank_theta)
     print(json.dumps(out))
 
