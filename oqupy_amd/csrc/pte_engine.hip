@@ -1047,6 +1047,7 @@ struct Batcher {
   int running = 0;
   std::vector<SvdRequest*> pending;
   int max_jacobi_grid = 0;
+  int tile_jacobi_grid = 0;   // co-resident grid of jacobi_tile_batch_kernel (0: off)
   int num_cus = 0;
   // statistics: served batches, requests, launches, host wall time serving
   long n_batches = 0, n_requests = 0, n_launches = 0;
@@ -1072,6 +1073,15 @@ struct Batcher {
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)Workspace::kSmallLdsMax));
     require(max_jacobi_grid > 0, PTE_ERR_DEVICE, "batched Jacobi kernel cannot be resident");
+    // the LDS-tile variant: one workgroup per compute unit
+    const size_t tile_lds = Workspace::tile_lds_bytes(kTileMaxRows);
+    HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_tile_batch_kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)tile_lds));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(jacobi_tile_batch_kernel), 256, tile_lds));
+    tile_jacobi_grid = std::min(per_cu * num_cus, 1024);
+    if (const char* v = std::getenv("PTE_JACOBI_TILE"))
+      if (std::atoi(v) == 0) tile_jacobi_grid = 0;
   }
 
   void destroy() {
@@ -1373,7 +1383,11 @@ struct Batcher {
   }
 
   void run_jacobi(std::vector<SvdRequest*>& jr) {
-    const int cap = max_jacobi_grid;
+    // the LDS-tile kernel when its grid holds the batch's block pairs; a
+    // chunk whose slices do not all fit the tile takes the streaming kernel
+    // (its grid within the same cap)
+    const bool tile_try = tile_jacobi_grid > 0;
+    const int cap = tile_try ? std::min(tile_jacobi_grid, max_jacobi_grid) : max_jacobi_grid;
     auto ming = [](SvdRequest* r) { return Workspace::jacobi_blocks(r->jn) / 2; };
     for (auto& ch : chunks(jr, cap, ming)) {
       std::vector<int> nat, mn;
@@ -1426,10 +1440,19 @@ struct Batcher {
       zero_items_kernel<<<(int)zeros.size(), 256, 0, stream>>>(
           reinterpret_cast<const ZeroItem*>(base + oz));
       void* kargs[] = {&da, &ds, &dm};
+      int tile_rows = 0;
+      for (const JacobiArgs& ja : args) tile_rows = std::max(tile_rows, ja.xrows_per + ja.vrows_per);
+      const bool tile = tile_try && tile_rows <= kTileMaxRows;
       HIP_CHECK(hipEventRecord(ev0, stream));
-      HIP_CHECK(hipLaunchCooperativeKernel(
-          reinterpret_cast<const void*>(jacobi_batch_kernel), dim3(total), dim3(256), kargs,
-          0, stream));
+      if (tile) {
+        HIP_CHECK(hipLaunchCooperativeKernel(
+            reinterpret_cast<const void*>(jacobi_tile_batch_kernel), dim3(total), dim3(256),
+            kargs, Workspace::tile_lds_bytes(tile_rows), stream));
+      } else {
+        HIP_CHECK(hipLaunchCooperativeKernel(
+            reinterpret_cast<const void*>(jacobi_batch_kernel), dim3(total), dim3(256), kargs,
+            0, stream));
+      }
       n_launches += 1;
       HIP_CHECK(hipEventRecord(ev1, stream));
       const unsigned np = (unsigned)in.size();

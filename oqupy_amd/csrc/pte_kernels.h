@@ -171,6 +171,8 @@ __global__ void jacobi_kernel(JacobiArgs a);
 // the same with the slice held in an LDS tile of xrows_per + vrows_per rows
 // (dynamic LDS: rows * kJacobiTileLd complex entries; at most kTileMaxRows rows)
 __global__ void jacobi_tile_kernel(JacobiArgs a);
+__global__ void jacobi_tile_batch_kernel(const JacobiArgs* probs, const int* starts,
+                                         const int* block_prob);
 __global__ void jacobi_batch_kernel(const JacobiArgs* probs, const int* starts,
                                     const int* block_prob);
 __global__ void column_norm_kernel(const double2* X, int m, int n, long ldx, double* sigma);

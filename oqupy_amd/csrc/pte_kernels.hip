@@ -1270,6 +1270,17 @@ jacobi_tile_kernel(JacobiArgs a) {
   jacobi_tile_body(a, blockIdx.x, gridDim.x);
 }
 
+// Several independent SVDs in one cooperative launch of the LDS-tile
+// Jacobi (problem map as in jacobi_batch_kernel; the dynamic LDS holds the
+// largest problem's tile).
+__global__ void __launch_bounds__(JT)
+jacobi_tile_batch_kernel(const JacobiArgs* probs, const int* starts, const int* block_prob) {
+  const int p = block_prob[blockIdx.x];
+  const int s0 = starts[p];
+  const JacobiArgs a = probs[p];
+  jacobi_tile_body(a, blockIdx.x - s0, starts[p + 1] - s0);
+}
+
 // sigma_j = || X[:, j] ||  (one workgroup per column)
 __device__ __forceinline__ void column_norm_body(const double2* X, int m, int n, long ldx,
                                                  double* sigma, int j) {
