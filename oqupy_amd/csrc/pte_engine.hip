@@ -363,6 +363,8 @@ struct Workspace {
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     max_coop_blocks = per_cu * prop.multiProcessorCount;
     num_cus = prop.multiProcessorCount;
+    require(prop.sharedMemPerBlock >= kSmallLdsMax + 1024, PTE_ERR_DEVICE,
+            "the engine needs 160 KiB of LDS per workgroup (gfx950)");
     init_tile_kernel();
     HIP_CHECK(hipMalloc((void**)&qr_cand, sizeof(QrcpCand) * 2 * 256));
     HIP_CHECK(hipMalloc((void**)&qr_rank_d, sizeof(int)));
@@ -464,6 +466,8 @@ struct Workspace {
     HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     max_coop_blocks = per_cu * prop.multiProcessorCount;
     num_cus = prop.multiProcessorCount;
+    require(prop.sharedMemPerBlock >= kSmallLdsMax + 1024, PTE_ERR_DEVICE,
+            "the engine needs 160 KiB of LDS per workgroup (gfx950)");
     init_tile_kernel();
     require(max_coop_blocks > 0, PTE_ERR_DEVICE, "Jacobi kernel cannot be resident");
     HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(small_svd_kernel),
@@ -835,13 +839,19 @@ struct Workspace {
   }
 
   // ---- small SVDs: one workgroup, LDS resident (small_svd_kernel) ----
-  static constexpr size_t kSmallLdsMax = 150 * 1024;
+  // dynamic LDS of small_svd_kernel: 160 KiB less 1 KiB for its static part
+  static constexpr size_t kSmallLdsMax = 159 * 1024;
+  // the QR phase needs M and 64 bytes per column; the Jacobi phase Y
+  // (n x r) and, when it fits, W (r x r), r <= min(m, n); the epilogue Y and
+  // two r-vectors
+  static size_t small_qr_lds(int m, int n) { return (size_t)m * n * 16 + (size_t)n * 64; }
   static size_t small_lds(int m, int n) {
     const size_t r = (size_t)std::min(m, n);
-    return std::max((size_t)m * n * 16 + (size_t)n * 20, ((size_t)n * r + r * r) * 16);
+    return std::max({small_qr_lds(m, n), (size_t)n * r * 16 + r * 16,
+                     std::min(kSmallLdsMax, ((size_t)n * r + r * r) * 16)});
   }
   bool small_ok(int m, int n) const {
-    return small_enabled && n <= kSmallMaxCols && small_lds(m, n) <= kSmallLdsMax;
+    return small_enabled && n <= kSmallMaxCols && small_qr_lds(m, n) <= kSmallLdsMax;
   }
 
   // buffers and arguments of small_svd_kernel on Q (m x n)
@@ -861,6 +871,7 @@ struct Workspace {
     sa.epsrel = epsrel; sa.max_rank = chi_cap; sa.max_sweeps = max_sweeps;
     sa.copy = copy;
     sa.stop_rel = 0.0;
+    sa.lds_bytes = (long)small_lds(m, n);
     return sa;
   }
 
@@ -881,7 +892,7 @@ struct Workspace {
     SmallSvdArgs sa = small_prepare(m, n, nullptr);
     if (adaptive) sa.stop_rel = rank->theta * epsrel;
     mark(evj0);
-    small_svd_kernel<<<1, 256, small_lds(m, n), svd_stream>>>(sa);
+    small_svd_kernel<<<1, kSmallThreads, small_lds(m, n), svd_stream>>>(sa);
     mark(evj1);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
@@ -1267,11 +1278,12 @@ struct Batcher {
       }
     }
     if (in.empty()) return;
+    for (auto& sa : args) sa.lds_bytes = (long)lds;
     Pack pk;
     const size_t oa = pk.add(args.data(), args.size() * sizeof(SmallSvdArgs));
     char* base = upload(pk);
     HIP_CHECK(hipEventRecord(ev0, stream));
-    small_svd_batch_kernel<<<(unsigned)in.size(), 256, lds, stream>>>(
+    small_svd_batch_kernel<<<(unsigned)in.size(), kSmallBatchThreads, lds, stream>>>(
         reinterpret_cast<const SmallSvdArgs*>(base + oa));
     HIP_CHECK(hipEventRecord(ev1, stream));
     HIP_CHECK(hipGetLastError());

@@ -140,9 +140,12 @@ struct SmallSvdArgs {
   double epsrel; int max_rank; int max_sweeps;
   SvdInfo* copy;    // optional second copy of the summary (batched gather)
   double stop_rel;  // > 0: stop the QR at residual <= stop_rel * |beta_0|
+  long lds_bytes;   // dynamic LDS of the launch (W stays in LDS when it fits)
 };
 
-constexpr int kSmallMaxCols = 64;           // columns of M in small_svd_kernel
+constexpr int kSmallMaxCols = 96;           // columns of M in small_svd_kernel
+constexpr int kSmallThreads = 1024;         // workgroup of small_svd_kernel
+constexpr int kSmallBatchThreads = 512;     // workgroup of small_svd_batch_kernel
 
 struct ZeroItem {
   unsigned* p; int words; int pad;

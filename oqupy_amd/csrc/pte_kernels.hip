@@ -31,9 +31,29 @@ __device__ unsigned long long g_phase[16];
       ph_t_ = t_;                                                      \
     }                                                                  \
   } while (0)
+// Segment cycle counts of the one-workgroup SVD (thread 0, s_memtime), in
+// the same build: SEG_INIT() starts, SEG(k) adds the cycles since the last
+// mark to slot k, SEG_FLUSH(k0, n) adds slots k0.. to g_phase.
+#define SEG_INIT()                                 \
+  unsigned long long sg_[6] = {0, 0, 0, 0, 0, 0}; \
+  unsigned long long sg_t_ = clock64()
+#define SEG(k)                                \
+  do {                                        \
+    const unsigned long long t_ = clock64(); \
+    sg_[k] += t_ - sg_t_;                     \
+    sg_t_ = t_;                               \
+  } while (0)
+#define SEG_FLUSH(k0, cnt)                                                 \
+  do {                                                                     \
+    if (threadIdx.x == 0)                                                  \
+      for (int k_ = 0; k_ < (cnt); ++k_) atomicAdd(&g_phase[(k0) + k_], sg_[k_]); \
+  } while (0)
 #else
 #define PHASE_INIT() (void)0
 #define PHASE(k) (void)0
+#define SEG_INIT() (void)0
+#define SEG(k) (void)0
+#define SEG_FLUSH(k0, cnt) (void)0
 #endif
 
 // Optional per-round timestamps of the Jacobi (thread 0 of every workgroup,
@@ -1936,13 +1956,17 @@ build_rh_batch_kernel(const RhArgs* pa) {
 }
 
 // ---------------------------------------------------------------------------
-// Small truncated SVD in one workgroup (M with at most kSmallMaxCols columns,
-// staged in LDS): the pivoted QR of qrcp_kernel (same reflectors, pivoting,
-// norm downdating and eps-rank stop), then the one-sided Jacobi of
-// Y = R^H (n x r) with scalar column pairs in the round-robin order (the
-// same rotation formulas and stopping rule as jacobi_kernel), then singular
-// values, order and cut. Barriers are workgroup barriers only; column pairs
-// (and QR columns) are reduced by lane groups with wavefront shuffles.
+// Small truncated SVD in one workgroup of NT threads (M with at most
+// kSmallMaxCols columns, staged in LDS): the pivoted QR of qrcp_kernel (same
+// reflectors, pivoting, norm downdating and eps-rank stop), then the
+// one-sided Jacobi of Y = R^H (n x r) with scalar column pairs in the
+// round-robin order (the rotation and stopping rule of jacobi_kernel), then
+// singular values, order and cut. Barriers are workgroup barriers only;
+// column pairs (and QR columns) are reduced by lane groups with wavefront
+// shuffles. Y stays in LDS; the accumulated rotation W (r x r) is kept in LDS
+// when Y and W fit in the dynamic LDS (a.lds_bytes), else in a.W, whose rows
+// a lane group loads at the start of its round so the loads overlap the
+// pair's inner products.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double group_sum(double v, int G) { return group_sum1(v, G); }
 
@@ -1952,28 +1976,166 @@ __device__ __forceinline__ int pow2_ceil(int x) {
   return p;
 }
 
+// Sum over an NT-thread workgroup with one __syncthreads (wavefront sums in
+// slot[parity][wave], parity alternating between calls); every thread gets
+// the same bits.
+template <int NT>
+__device__ __forceinline__ double block_sum_nt(double v, double (*slot)[NT / 64], int& parity) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) slot[parity][threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) r += slot[parity][w];
+  parity ^= 1;
+  return r;
+}
+
+// Cyclic one-sided Jacobi sweeps on the columns of Ys (n x r, ld n, LDS),
+// rotations accumulated in W (r x r, ld r): in LDS after Y when WL, else Wg.
+// Lane group q (GJ lanes) owns pair q of each round; a lane holds rows
+// gj, gj + GJ, ... (at most RP) of the pair's two columns in registers. The
+// row loops are branch-free: a block of GJ rows is skipped when it starts
+// past the end (a test uniform over the workgroup), and inside the last block
+// a lane past the end works on the last row instead, masked out of the inner
+// products; it then stores the same bits as the lane that owns that row.
+// Returns the sweep count; `converged` when the last sweep rotated nothing.
+template <int NT, bool WL>
+__device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int r,
+                                            double tol2, double abs_tol2, int max_sweeps,
+                                            bool& converged) {
+  constexpr int RP = kSmallMaxCols * 64 / NT;   // GJ >= NT / 64 lanes per pair
+  double2* const Wp = WL ? Ys + (long)n * r : Wg;
+  const int tid = threadIdx.x;
+  const int re = r + (r & 1);
+  const int P = re / 2;
+  const int GJ = min(min(NT / pow2_ceil(P), 64), pow2_ceil(n));
+  const int q = tid / GJ, gj = tid % GJ;
+  const int L = re > 1 ? re - 1 : 1;
+  // a wavefront whose lane groups all lie beyond the last pair only takes
+  // part in the barriers
+  const bool wave_busy = (tid & ~63) / GJ < P;
+  const double2 zero = make_double2(0.0, 0.0);
+  int sweeps = 0;
+  converged = false;
+  SEG_INIT();
+  for (; sweeps < max_sweeps; ++sweeps) {
+    int any = 0;
+    for (int round = 0; round < L; ++round) {
+      if (!wave_busy) {
+        __syncthreads();
+        continue;
+      }
+      int ci = 0, cj = 0;
+      bool valid = false;
+      if (q < P && re > 1) {
+        // round-robin pairing of tournament_pair without integer division
+        const int Lr = re - 1;
+        if (q == 0) {
+          ci = Lr;
+          cj = round;
+        } else {
+          ci = round + q;
+          if (ci >= Lr) ci -= Lr;
+          cj = round - q;
+          if (cj < 0) cj += Lr;
+        }
+        if (ci > cj) { const int tt = ci; ci = cj; cj = tt; }
+        valid = cj < r;
+      }
+      if (!valid) { ci = 0; cj = 0; }   // harmless addresses; no rotation
+      double2* y0 = Ys + (long)ci * n;
+      double2* y1 = Ys + (long)cj * n;
+      double2 yi[RP], yj[RP];
+      double al = 0.0, be = 0.0;
+      double2 ga = zero;
+#pragma unroll
+      for (int t = 0; t < RP; ++t) {
+        if (t * GJ < n) {
+          const int i0 = gj + t * GJ;
+          const int i = min(i0, n - 1);
+          yi[t] = y0[i];
+          yj[t] = y1[i];
+          const double2 u = i0 < n ? yi[t] : zero;
+          const double2 v = i0 < n ? yj[t] : zero;
+          al += cabs2(u);
+          be += cabs2(v);
+          cfmac(ga, u, v);
+        }
+      }
+      SEG(0);
+      group_sum4(al, be, ga.x, ga.y, GJ);
+      SEG(1);
+      double rot[4];
+      const bool rotate = jacobi_rotation_fast(valid, al, be, ga, tol2, abs_tol2, rot);
+      SEG(2);
+      if (rotate) {
+        any = 1;
+        const double c = rot[0], sn = rot[1];
+        const double2 cph = make_double2(rot[2], -rot[3]);
+#pragma unroll
+        for (int t = 0; t < RP; ++t) {
+          if (t * GJ < n) {
+            const int i = min(gj + t * GJ, n - 1);
+            const double2 xp = yi[t];
+            const double2 xq = cmul(yj[t], cph);
+            y0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+            y1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+          }
+        }
+        SEG(3);
+        double2* w0 = Wp + (long)ci * r;
+        double2* w1 = Wp + (long)cj * r;
+#pragma unroll
+        for (int t = 0; t < RP; ++t) {
+          if (t * GJ < r) {
+            const int i = min(gj + t * GJ, r - 1);
+            const double2 xp = w0[i];
+            const double2 xq = cmul(w1[i], cph);
+            w0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+            w1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+          }
+        }
+        SEG(4);
+      }
+      __syncthreads();
+      SEG(5);
+    }
+    if (!__syncthreads_or(any)) {
+      converged = true;
+      break;
+    }
+  }
+  SEG_FLUSH(0, 6);
+  return sweeps;
+}
+
+template <int NT>
 __device__ void small_svd_body(const SmallSvdArgs& a) {
+  static_assert(NT % 64 == 0 && (kSmallMaxCols * 64) % NT == 0, "workgroup size");
   extern __shared__ double2 lds_small[];
-  __shared__ double slot[2][4];
-  __shared__ int q_piv[kSmallMaxCols];
-  __shared__ double2 q_tau[kSmallMaxCols], q_vscale[kSmallMaxCols];
-  __shared__ double q_beta[kSmallMaxCols];
+  __shared__ double slot[2][NT / 64];
   int parity = 0;
   const int m = a.m, n = a.n, tid = threadIdx.x, lane = tid & 63;
-  double2* As = lds_small;                                  // m x n, ld m
-  double* vn1 = reinterpret_cast<double*>(As + (long)m * n);
+  // QR phase: M (m x n, ld m), then the per-column scalars (64 n bytes)
+  double2* As = lds_small;
+  double2* q_tau = As + (long)m * n;
+  double2* q_vscale = q_tau + n;
+  double* vn1 = reinterpret_cast<double*>(q_vscale + n);
   double* vn2 = vn1 + n;
-  int* done = reinterpret_cast<int*>(vn2 + n);
+  double* q_beta = vn2 + n;
+  int* done = reinterpret_cast<int*>(q_beta + n);
+  int* q_piv = done + n;
   const double eps = 2.220446049250313e-16;
 
   PHASE_INIT();
-  for (long e = tid; e < (long)m * n; e += 256) As[e] = a.A[e];
+  for (long e = tid; e < (long)m * n; e += NT) As[e] = a.A[e];
   __syncthreads();
   PHASE(6);
   // ---- pivoted QR in LDS ----
   const int NP = pow2_ceil(n);
-  // threads per column: at most 64, at most 256 / NP, no more than the rows
-  const int G = min(min(NP >= 4 ? 256 / NP : 64, pow2_ceil(m)), 64);
+  // threads per column: at most 64, at most NT / NP, no more than the rows
+  const int G = min(min(NP >= NT / 64 ? NT / NP : 64, pow2_ceil(m)), 64);
   const int myc = tid / G, g = tid % G;
   {
     double part = 0.0;
@@ -1983,11 +2145,26 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     if (myc < n && g == 0) { vn1[myc] = nrm; vn2[myc] = nrm; done[myc] = 0; }
   }
   __syncthreads();
-  // pivot (largest residual norm, ties to the smaller column) and residual
-  // Frobenius norm squared, computed identically by every wavefront
-  int p;
-  double total;
-  auto choose = [&]() {
+  // Each step takes two barriers: after the first, every wavefront reads the
+  // step's pivot and reflector from LDS and the group of each remaining
+  // column applies the reflector to it and downdates its norm; after the
+  // second, wavefront 0 alone chooses the next pivot (largest residual norm,
+  // ties to the smaller column), decides whether to stop, computes the next
+  // reflector and stores it into the pivot column already scaled (v_i =
+  // M_ij * scale below the diagonal, so vscale is 1), so the scalar work is
+  // not repeated by every wavefront.
+  __shared__ int s_ctl[2];          // pivot column, 1 while the QR continues
+  __shared__ double2 s_tau;
+  const int wave = tid >> 6;
+  int j = 0;
+  // adaptive stop (stop_rel > 0): once the residual is at most
+  // stop_rel * |beta_0|; the residual is then counted in the cut (folded2,
+  // used by thread 0 only)
+  double drop2 = 0.0, stop2 = -1.0, folded2 = 0.0;
+  const double tol3z = 1.4901161193847656e-08;
+  const int kmax = m < n ? m : n;
+  // wavefront 0: pivot and reflector of step j, or the stop
+  auto pivot_step = [&]() {
     double bv = -1.0, sv = 0.0;
     int bc = -1;
     for (int c = lane; c < n; c += 64)
@@ -1998,45 +2175,62 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     better(bv, bc, dpp_d<kDppMirror>(bv), dpp_i<kDppMirror>(bc));
     better(bv, bc, __shfl_xor(bv, 16, 64), __shfl_xor(bc, 16, 64));
     better(bv, bc, __shfl_xor(bv, 32, 64), __shfl_xor(bc, 32, 64));
-    p = bc;
-    total = wave_sum(sv);
-  };
-  choose();
-  const double drop2 = eps * eps * total;
-  const double tol3z = 1.4901161193847656e-08;
-  const int kmax = m < n ? m : n;
-  int j = 0;
-  // adaptive stop (stop_rel > 0): once the residual is at most
-  // stop_rel * |beta_0|; the residual is then counted in the cut
-  double stop2 = -1.0, folded2 = 0.0;
-  while (j < kmax && p >= 0 && total > drop2) {
-    const double2* pc = As + (long)p * m;
-    // every wavefront sums the pivot column itself (no workgroup barrier)
-    double part = 0.0;
-    for (int i = j + 1 + lane; i < m; i += 64) part += cabs2(pc[i]);
-    const double xnorm2 = wave_sum(part);
-    const double2 alpha = pc[j];
-    double2 tau, scale;
-    double beta;
-    if (xnorm2 == 0.0 && alpha.y == 0.0) {
-      tau = make_double2(0.0, 0.0);
-      scale = make_double2(0.0, 0.0);
-      beta = alpha.x;
+    const int p = bc;
+    const double total = wave_sum(sv);
+    if (j == 0) drop2 = eps * eps * total;
+    bool go;
+    if (a.stop_rel > 0.0 && j >= 1) {
+      if (j == 1) stop2 = (a.stop_rel * q_beta[0]) * (a.stop_rel * q_beta[0]);
+      if (total <= stop2) {
+        folded2 = (p >= 0 && j < kmax) ? total : 0.0;
+        go = false;
+      } else {
+        go = j < kmax && p >= 0 && total > drop2;
+      }
     } else {
-      beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
-      const double dr = alpha.x - beta, di = alpha.y;
-      const double den = dr * dr + di * di;
-      scale = make_double2(dr / den, -di / den);
-      tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      go = j < kmax && p >= 0 && total > drop2;
     }
-    if (tid == 0) {
-      // kept in LDS until the end: a global store here would be waited for
-      // at every barrier
-      q_piv[j] = p;
-      q_tau[j] = tau;
-      q_vscale[j] = scale;
-      q_beta[j] = beta;
+    if (go) {
+      double2* pc = As + (long)p * m;
+      double part = 0.0;
+      for (int i = j + 1 + lane; i < m; i += 64) part += cabs2(pc[i]);
+      const double xnorm2 = wave_sum(part);
+      const double2 alpha = pc[j];
+      double2 tau, scale;
+      double beta;
+      if (xnorm2 == 0.0 && alpha.y == 0.0) {
+        tau = make_double2(0.0, 0.0);
+        scale = make_double2(0.0, 0.0);
+        beta = alpha.x;
+      } else {
+        beta = -copysign(sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2), alpha.x);
+        const double dr = alpha.x - beta, di = alpha.y;
+        const double den = dr * dr + di * di;
+        scale = make_double2(dr / den, -di / den);
+        tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      }
+      for (int i = j + 1 + lane; i < m; i += 64) pc[i] = cmul(pc[i], scale);
+      if (lane == 0) {
+        q_piv[j] = p;
+        q_tau[j] = tau;
+        q_vscale[j] = make_double2(1.0, 0.0);
+        q_beta[j] = beta;
+        s_tau = tau;
+        s_ctl[0] = p;
+      }
     }
+    if (lane == 0) s_ctl[1] = go ? 1 : 0;
+  };
+  SEG_INIT();
+  if (wave == 0) pivot_step();
+  SEG(0);
+  for (;;) {
+    __syncthreads();
+    SEG(1);
+    if (!s_ctl[1]) break;
+    const int p = s_ctl[0];
+    const double2 tau = s_tau;
+    const double2* pv = As + (long)p * m;   // reflector, v_j = 1 implicit
     const bool reflect = (tau.x != 0.0 || tau.y != 0.0);
     // the group of column myc applies H_j^H and downdates the column norm;
     // only that group touches the column, its norms and its done flag
@@ -2050,7 +2244,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
           double2 w = make_double2(0.0, 0.0);
 #pragma unroll 4
           for (int i = j + g; i < m; i += G) {
-            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : pv[i];
             cfmac(w, vi, col[i]);
           }
           {
@@ -2060,7 +2254,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
           const double2 tw = cmul(make_double2(tau.x, -tau.y), w);
 #pragma unroll 4
           for (int i = j + g; i < m; i += G) {
-            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : cmul(pc[i], scale);
+            const double2 vi = (i == j) ? make_double2(1.0, 0.0) : pv[i];
             col[i] = csub(col[i], cmul(vi, tw));
           }
           // row j of the column after the reflection (held by lane g = 0)
@@ -2083,18 +2277,13 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
         }
       }
     }
+    SEG(2);
     __syncthreads();
-    choose();
     ++j;
-    if (a.stop_rel > 0.0) {
-      if (j == 1) stop2 = (a.stop_rel * beta) * (a.stop_rel * beta);
-      // `total` is computed identically by every wavefront: a uniform exit
-      if (total <= stop2) {
-        folded2 = (p >= 0 && j < kmax) ? total : 0.0;
-        break;
-      }
-    }
+    if (wave == 0) pivot_step();
+    SEG(3);
   }
+  SEG_FLUSH(8, 4);
   const int r = j;
   PHASE(13);
   if (tid == 0) {
@@ -2104,7 +2293,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     *a.qrank = r;
   }
   __syncthreads();
-  for (int t = tid; t < n; t += 256) {
+  for (int t = tid; t < n; t += NT) {
     a.piv[t] = q_piv[t];
     if (t < r) {
       a.tau[t] = q_tau[t];
@@ -2113,7 +2302,7 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     }
   }
   // reflectors and R back to global memory (apply_q_kernel reads them)
-  for (long e = tid; e < (long)m * n; e += 256) a.A[e] = As[e];
+  for (long e = tid; e < (long)m * n; e += NT) a.A[e] = As[e];
   __syncthreads();
   if (r == 0) {
     if (tid == 0) {
@@ -2125,10 +2314,14 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     }
     return;
   }
-  // ---- Y = R^H (n x r, ld n) and W = I (r x r) in LDS ----
+  // ---- Y = R^H (n x r, ld n) in LDS; W = I (r x r) in LDS or a.W ----
+  // Y lies inside the space of M (r <= m), below the scalars it reads; W
+  // (when it fits) follows Y and may cover the scalars, so it is set after
+  // a barrier.
   double2* Ys = lds_small;
-  double2* Wsm = Ys + (long)n * r;
-  for (long e = tid; e < (long)n * r; e += 256) {
+  const bool wl = (long)(n + r) * r * (long)sizeof(double2) <= a.lds_bytes;
+  double2* Wp = wl ? Ys + (long)n * r : a.W;
+  for (long e = tid; e < (long)n * r; e += NT) {
     const int sidx = e % n, t = e / n;
     const int ps = q_piv[sidx];
     double2 v = make_double2(0.0, 0.0);
@@ -2137,87 +2330,29 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
     Ys[e] = v;
   }
   __syncthreads();
-  for (long e = tid; e < (long)r * r; e += 256)
-    Wsm[e] = make_double2((e % r) == (e / r) ? 1.0 : 0.0, 0.0);
+  for (long e = tid; e < (long)r * r; e += NT)
+    Wp[e] = make_double2((e % r) == (e / r) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
   double fpart = 0.0;
-  for (long e = tid; e < (long)n * r; e += 256) fpart += cabs2(Ys[e]);
-  const double fro = sqrt(block_sum256(fpart, slot, parity));
+  for (long e = tid; e < (long)n * r; e += NT) fpart += cabs2(Ys[e]);
+  const double fro = sqrt(block_sum_nt<NT>(fpart, slot, parity));
   const double tol = (n > 16 ? n : 16) * eps, abs_tol = eps * fro;
-  const double tol2 = tol * tol, abs_tol2 = abs_tol * abs_tol;
   // ---- one-sided Jacobi on the columns of Y ----
-  const int re = r + (r & 1);
-  const int P = re / 2;
-  const int PG = pow2_ceil(P);
-  const int GJ = min(min(PG >= 4 ? 256 / PG : 64, pow2_ceil(n)), 64);   // threads per pair
-  const int q = tid / GJ, gj = tid % GJ;
-  int sweeps = 0;
   bool converged = false;
-  for (; sweeps < a.max_sweeps; ++sweeps) {
-    int any = 0;
-    for (int round = 0; round < (re > 1 ? re - 1 : 1); ++round) {
-      int ci = -1, cj = -1;
-      if (q < P && re > 1) {
-        // round-robin pairing of tournament_pair without integer division
-        const int L = re - 1;
-        if (q == 0) {
-          ci = L;
-          cj = round;
-        } else {
-          ci = round + q;
-          if (ci >= L) ci -= L;
-          cj = round - q;
-          if (cj < 0) cj += L;
-        }
-        if (ci > cj) { const int tt = ci; ci = cj; cj = tt; }
-        if (cj >= r) { ci = -1; cj = -1; }
-      }
-      if (ci >= 0) {
-        double2* yi = Ys + (long)ci * n;
-        double2* yj = Ys + (long)cj * n;
-        double al = 0.0, be = 0.0;
-        double2 ga = make_double2(0.0, 0.0);
-#pragma unroll 4
-        for (int i = gj; i < n; i += GJ) {
-          al += cabs2(yi[i]);
-          be += cabs2(yj[i]);
-          cfmac(ga, yi[i], yj[i]);
-        }
-        group_sum4(al, be, ga.x, ga.y, GJ);
-        double rot[4];
-        if (jacobi_rotation(al, be, ga, tol2, abs_tol2, rot)) {
-          any = 1;
-          const double c = rot[0], sn = rot[1];
-          const double2 cph = make_double2(rot[2], -rot[3]);
-#pragma unroll 4
-          for (int i = gj; i < n; i += GJ) {
-            const double2 xp = yi[i];
-            const double2 xq = cmul(yj[i], cph);
-            yi[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
-            yj[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
-          }
-          double2* wi = Wsm + (long)ci * r;
-          double2* wj = Wsm + (long)cj * r;
-          for (int i = gj; i < r; i += GJ) {
-            const double2 xp = wi[i];
-            const double2 xq = cmul(wj[i], cph);
-            wi[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
-            wj[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (!__syncthreads_or(any)) {
-      converged = true;
-      break;
-    }
-  }
+  const int sweeps = wl ? small_jacobi<NT, true>(Ys, a.W, n, r, tol * tol, abs_tol * abs_tol,
+                                                 a.max_sweeps, converged)
+                        : small_jacobi<NT, false>(Ys, a.W, n, r, tol * tol, abs_tol * abs_tol,
+                                                  a.max_sweeps, converged);
   PHASE(14);
   // ---- singular values, order, cut; Z and W to global memory ----
-  for (long e = tid; e < (long)n * r; e += 256) a.Y[e] = Ys[e];
-  for (long e = tid; e < (long)r * r; e += 256) a.W[e] = Wsm[e];
-  __shared__ double s_sig[kSmallMaxCols], s_sorted[kSmallMaxCols];
-  for (int c = tid / 64; c < r; c += 4) {
+  for (long e = tid; e < (long)n * r; e += NT) a.Y[e] = Ys[e];
+  if (wl)
+    for (long e = tid; e < (long)r * r; e += NT) a.W[e] = Wp[e];
+  __syncthreads();
+  // two r-vectors after Y (over W, which is in global memory now)
+  double* s_sig = reinterpret_cast<double*>(Ys + (long)n * r);
+  double* s_sorted = s_sig + r;
+  for (int c = tid / 64; c < r; c += NT / 64) {
     double part2 = 0.0;
     for (int i = lane; i < n; i += 64) part2 += cabs2(Ys[i + (long)c * n]);
     const double sg = sqrt(wave_sum(part2));
@@ -2261,15 +2396,15 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   PHASE(15);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSmallThreads)
 small_svd_kernel(SmallSvdArgs a) {
-  small_svd_body(a);
+  small_svd_body<kSmallThreads>(a);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kSmallBatchThreads)
 small_svd_batch_kernel(const SmallSvdArgs* probs) {
   const SmallSvdArgs a = probs[blockIdx.x];
-  small_svd_body(a);
+  small_svd_body<kSmallBatchThreads>(a);
 }
 
 // Left singular vectors Q W[:, perm[j]] (j < k) of the pivoted QR, the

@@ -45,11 +45,14 @@ def _graded(m, n, seed, decades=13.3, eps=1e-7):
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (4, 4), (5, 3), (3, 5), (40, 30),
-                                 (30, 40), (64, 64), (100, 90), (260, 210),
+                                 (30, 40), (64, 64), (70, 88), (100, 90),
+                                 (96, 96), (150, 60), (260, 210),
                                  (4000, 600), (8192, 1024)])
 def test_split_svd_matches_lapack(m, n):
     """Every shape up to 9216 rows takes the pivoted-QR path (the two tall
-    cases included); singular values, product and orthonormality against
+    cases included). Up to 96 columns with M in LDS the whole SVD runs in one
+    workgroup: (64, 64) keeps the rotations in LDS, (100, 90) and (96, 96)
+    in global memory. Singular values, product and orthonormality against
     LAPACK."""
     _announce(f"split_svd {m}x{n}")
     from oracle.chain import truncated_svd
