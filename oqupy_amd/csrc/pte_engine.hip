@@ -891,14 +891,16 @@ struct Workspace {
   int svd_small(int m, int n, bool adaptive = false) {
     SmallSvdArgs sa = small_prepare(m, n, nullptr);
     if (adaptive) sa.stop_rel = rank->theta * epsrel;
-    mark(evj0);
+    // timed (two event records on the stream) only for the SVD trace
+    const bool timed = trace != nullptr;
+    if (timed) mark(evj0);
     small_svd_kernel<<<1, kSmallThreads, small_lds(m, n), svd_stream>>>(sa);
-    mark(evj1);
+    if (timed) mark(evj1);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(info_h, info_d, sizeof(SvdInfo), hipMemcpyDeviceToHost,
                              svd_stream));
     HIP_CHECK(hipStreamSynchronize(svd_stream));
-    return small_finish(m, n, elapsed(evj0, evj1));
+    return small_finish(m, n, timed ? elapsed(evj0, evj1) : 0.f);
   }
 
   int qr_natural_grid(int n) const { return std::max(1, std::min({n, qr_blocks, num_cus})); }
@@ -969,7 +971,10 @@ struct Workspace {
     aq.B = B;
     aq.dj0 = dj0; aq.djk = djk; aq.da0 = da0; aq.dak = dak; aq.db0 = db0; aq.dbk = dbk;
     aq.dst = dst;
-    apply_q_kernel<<<std::min(k, 1024), 256, (size_t)qr_m * sizeof(double2), stream>>>(aq);
+    if (qr_m <= kApplyQWaveRows)
+      apply_q_wave_kernel<<<(k + 3) / 4, 256, 0, stream>>>(aq);
+    else
+      apply_q_kernel<<<std::min(k, 1024), 256, (size_t)qr_m * sizeof(double2), stream>>>(aq);
   }
 
   // S V^H rows of the last svd_qrcp: column c = a*B + b of M goes to

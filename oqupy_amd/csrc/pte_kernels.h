@@ -165,6 +165,8 @@ __global__ void small_svd_kernel(SmallSvdArgs a);
 __global__ void small_svd_batch_kernel(const SmallSvdArgs* probs);
 __global__ void gather_qr_rank_kernel(const QrcpArgs* probs, int n, int* out);
 __global__ void apply_q_kernel(ApplyQArgs a);
+__global__ void apply_q_wave_kernel(ApplyQArgs a);   // Q of at most kApplyQWaveRows rows
+constexpr int kApplyQWaveRows = 128;
 __global__ void zgemm_kernel(GemmArgs a);
 __global__ void zgemm_mfma_kernel(GemmArgs a);
 __global__ void cgemm_mfma_kernel(GemmArgs a);

@@ -1999,11 +1999,16 @@ __device__ __forceinline__ double block_sum_nt(double v, double (*slot)[NT / 64]
 // past the end (a test uniform over the workgroup), and inside the last block
 // a lane past the end works on the last row instead, masked out of the inner
 // products; it then stores the same bits as the lane that owns that row.
+// A pair is skipped when neither column has been rotated since the round
+// one sweep earlier (the order repeats every sweep, so that round checked
+// this pair and found it converged): recomputing it would give the same bits.
+// last_rot[c] (LDS) is the round of column c's last rotation; a wavefront
+// whose pairs are all skipped only takes part in the barrier.
 // Returns the sweep count; `converged` when the last sweep rotated nothing.
 template <int NT, bool WL>
 __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int r,
                                             double tol2, double abs_tol2, int max_sweeps,
-                                            bool& converged) {
+                                            int* last_rot, bool& converged) {
   constexpr int RP = kSmallMaxCols * 64 / NT;   // GJ >= NT / 64 lanes per pair
   double2* const Wp = WL ? Ys + (long)n * r : Wg;
   const int tid = threadIdx.x;
@@ -2022,6 +2027,7 @@ __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int
   for (; sweeps < max_sweeps; ++sweeps) {
     int any = 0;
     for (int round = 0; round < L; ++round) {
+      const int gr = sweeps * L + round;   // global round index
       if (!wave_busy) {
         __syncthreads();
         continue;
@@ -2042,61 +2048,67 @@ __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int
         }
         if (ci > cj) { const int tt = ci; ci = cj; cj = tt; }
         valid = cj < r;
+        if (valid && gr >= L && last_rot[ci] < gr - L && last_rot[cj] < gr - L)
+          valid = false;   // converged at its last check, columns unchanged
       }
-      if (!valid) { ci = 0; cj = 0; }   // harmless addresses; no rotation
-      double2* y0 = Ys + (long)ci * n;
-      double2* y1 = Ys + (long)cj * n;
-      double2 yi[RP], yj[RP];
-      double al = 0.0, be = 0.0;
-      double2 ga = zero;
-#pragma unroll
-      for (int t = 0; t < RP; ++t) {
-        if (t * GJ < n) {
-          const int i0 = gj + t * GJ;
-          const int i = min(i0, n - 1);
-          yi[t] = y0[i];
-          yj[t] = y1[i];
-          const double2 u = i0 < n ? yi[t] : zero;
-          const double2 v = i0 < n ? yj[t] : zero;
-          al += cabs2(u);
-          be += cabs2(v);
-          cfmac(ga, u, v);
-        }
-      }
-      SEG(0);
-      group_sum4(al, be, ga.x, ga.y, GJ);
-      SEG(1);
-      double rot[4];
-      const bool rotate = jacobi_rotation_fast(valid, al, be, ga, tol2, abs_tol2, rot);
-      SEG(2);
-      if (rotate) {
-        any = 1;
-        const double c = rot[0], sn = rot[1];
-        const double2 cph = make_double2(rot[2], -rot[3]);
+      // a wavefront whose pairs are all skipped only takes part in the barrier
+      if (__ballot(valid) != 0) {
+        if (!valid) { ci = 0; cj = 0; }   // harmless addresses; no rotation
+        double2* y0 = Ys + (long)ci * n;
+        double2* y1 = Ys + (long)cj * n;
+        double2 yi[RP], yj[RP];
+        double al = 0.0, be = 0.0;
+        double2 ga = zero;
 #pragma unroll
         for (int t = 0; t < RP; ++t) {
           if (t * GJ < n) {
-            const int i = min(gj + t * GJ, n - 1);
-            const double2 xp = yi[t];
-            const double2 xq = cmul(yj[t], cph);
-            y0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
-            y1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+            const int i0 = gj + t * GJ;
+            const int i = min(i0, n - 1);
+            yi[t] = y0[i];
+            yj[t] = y1[i];
+            const double2 u = i0 < n ? yi[t] : zero;
+            const double2 v = i0 < n ? yj[t] : zero;
+            al += cabs2(u);
+            be += cabs2(v);
+            cfmac(ga, u, v);
           }
         }
-        SEG(3);
-        double2* w0 = Wp + (long)ci * r;
-        double2* w1 = Wp + (long)cj * r;
+        SEG(0);
+        group_sum4(al, be, ga.x, ga.y, GJ);
+        SEG(1);
+        double rot[4];
+        const bool rotate = jacobi_rotation_fast(valid, al, be, ga, tol2, abs_tol2, rot);
+        SEG(2);
+        if (rotate) {
+          any = 1;
+          const double c = rot[0], sn = rot[1];
+          const double2 cph = make_double2(rot[2], -rot[3]);
 #pragma unroll
-        for (int t = 0; t < RP; ++t) {
-          if (t * GJ < r) {
-            const int i = min(gj + t * GJ, r - 1);
-            const double2 xp = w0[i];
-            const double2 xq = cmul(w1[i], cph);
-            w0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
-            w1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+          for (int t = 0; t < RP; ++t) {
+            if (t * GJ < n) {
+              const int i = min(gj + t * GJ, n - 1);
+              const double2 xp = yi[t];
+              const double2 xq = cmul(yj[t], cph);
+              y0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+              y1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+            }
           }
+          SEG(3);
+          double2* w0 = Wp + (long)ci * r;
+          double2* w1 = Wp + (long)cj * r;
+#pragma unroll
+          for (int t = 0; t < RP; ++t) {
+            if (t * GJ < r) {
+              const int i = min(gj + t * GJ, r - 1);
+              const double2 xp = w0[i];
+              const double2 xq = cmul(w1[i], cph);
+              w0[i] = make_double2(c * xp.x - sn * xq.x, c * xp.y - sn * xq.y);
+              w1[i] = make_double2(sn * xp.x + c * xq.x, sn * xp.y + c * xq.y);
+            }
+          }
+          if (gj == 0) { last_rot[ci] = gr; last_rot[cj] = gr; }
+          SEG(4);
         }
-        SEG(4);
       }
       __syncthreads();
       SEG(5);
@@ -2338,11 +2350,14 @@ __device__ void small_svd_body(const SmallSvdArgs& a) {
   const double fro = sqrt(block_sum_nt<NT>(fpart, slot, parity));
   const double tol = (n > 16 ? n : 16) * eps, abs_tol = eps * fro;
   // ---- one-sided Jacobi on the columns of Y ----
+  __shared__ int last_rot[kSmallMaxCols];
+  for (int c = tid; c < r; c += NT) last_rot[c] = -(1 << 30);
+  __syncthreads();
   bool converged = false;
   const int sweeps = wl ? small_jacobi<NT, true>(Ys, a.W, n, r, tol * tol, abs_tol * abs_tol,
-                                                 a.max_sweeps, converged)
+                                                 a.max_sweeps, last_rot, converged)
                         : small_jacobi<NT, false>(Ys, a.W, n, r, tol * tol, abs_tol * abs_tol,
-                                                  a.max_sweeps, converged);
+                                                  a.max_sweeps, last_rot, converged);
   PHASE(14);
   // ---- singular values, order, cut; Z and W to global memory ----
   for (long e = tid; e < (long)n * r; e += NT) a.Y[e] = Ys[e];
@@ -2484,6 +2499,54 @@ apply_q_kernel(ApplyQArgs a) {
     }
     __syncthreads();
   }
+}
+
+// The same for Q of at most 128 rows: one wavefront per column, rows lane and
+// lane + 64 in registers, wavefront reductions only (no barrier, no LDS);
+// reflector t - 1 is loaded while step t is reduced.
+__global__ void __launch_bounds__(256)
+apply_q_wave_kernel(ApplyQArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = a.info->rank;
+  if (j >= k) return;   // a whole wavefront; the kernel has no barrier
+  const int r = *a.qrank;
+  const int m = a.m;
+  const double2 zero = make_double2(0.0, 0.0);
+  const int i0 = lane, i1 = lane + 64;
+  const double2* w = a.W + (long)a.perm[j] * a.ldw;
+  double2 x0 = i0 < r ? w[i0] : zero;
+  double2 x1 = i1 < r ? w[i1] : zero;
+  double2 n0 = zero, n1 = zero;
+  auto fetch = [&](int t) {
+    const double2* v = a.A + (long)a.piv[t] * a.lda;
+    n0 = i0 < m ? v[i0] : zero;
+    n1 = i1 < m ? v[i1] : zero;
+  };
+  if (r > 0) fetch(r - 1);
+  for (int t = r - 1; t >= 0; --t) {
+    const double2 c0 = n0, c1 = n1;
+    const double2 tau = a.tau[t];
+    const double2 sc = a.vscale[t];
+    if (t > 0) fetch(t - 1);
+    if (tau.x == 0.0 && tau.y == 0.0) continue;
+    const double2 one = make_double2(1.0, 0.0);
+    const double2 v0 = i0 == t ? one : (i0 > t ? cmul(c0, sc) : zero);
+    const double2 v1 = i1 == t ? one : (i1 > t && i1 < m ? cmul(c1, sc) : zero);
+    double2 acc = zero;
+    cfmac(acc, v0, x0);
+    cfmac(acc, v1, x1);
+    double z0 = 0.0, z1 = 0.0;
+    group_sum4(acc.x, acc.y, z0, z1, 64);
+    const double2 tw = cmul(tau, acc);
+    x0 = csub(x0, cmul(v0, tw));
+    x1 = csub(x1, cmul(v1, tw));
+  }
+  const long dj = a.dj0 + a.djk * (long)k;
+  const long da = a.da0 + a.dak * (long)k;
+  const long db = a.db0 + a.dbk * (long)k;
+  if (i0 < m) a.dst[(long)j * dj + (long)(i0 / a.B) * da + (long)(i0 % a.B) * db] = x0;
+  if (i1 < m) a.dst[(long)j * dj + (long)(i1 / a.B) * da + (long)(i1 % a.B) * db] = x1;
 }
 
 // X (rows x cols, column-major, ld = rows) = conj(src) with src row-major
