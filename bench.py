@@ -43,9 +43,9 @@ PHASE_STEPS = {"c2": 40, "c3": 0, "c5": 0}
 # what the counters read for a plain copy (keyed by config, precision, rank
 # form, phase steps, warm-up, timed steps)
 TRAFFIC = {
-    # FETCH_SIZE 1330705.0 KiB x 2.000 + WRITE_SIZE 1077194.8 KiB x 1.000 per
-    # Jacobi dispatch, mean over the window's 6760 dispatches
-    ("c2", "c128", "exact", 40, 5, 20): 3828331366,
+    # FETCH_SIZE 1366370.4 KiB x 2.000 + WRITE_SIZE 1104523.3 KiB x 1.000 per
+    # Jacobi dispatch, mean over the window's 6580 dispatches
+    ("c2", "c128", "exact", 40, 5, 20): 3929358419,
 }
 TRAFFIC_SOURCE = "profiles/hbm_traffic_svd_kernels_pt_tempo_dkmax200_steps47-66.md"
 
