@@ -2005,6 +2005,25 @@ __device__ __forceinline__ double block_sum_nt(double v, double (*slot)[NT / 64]
 // last_rot[c] (LDS) is the round of column c's last rotation; a wavefront
 // whose pairs are all skipped only takes part in the barrier.
 // Returns the sweep count; `converged` when the last sweep rotated nothing.
+// Lanes per pair (a power of two, NT/64..64, all pairs in one pass): the
+// count that minimises the issue slots of the most loaded SIMD, busy
+// wavefronts per SIMD x instructions per wavefront (rows per lane of the two
+// Y and two W columns, plus the reduction and the rotation); ties go to more
+// lanes (shorter chains).
+template <int NT>
+__device__ __forceinline__ int small_pair_lanes(int P, int n, int r) {
+  int best = NT / 64, best_cost = 1 << 30;
+  for (int gj = NT / 64; gj <= 64; gj *= 2) {
+    if (P * gj > NT || (gj > NT / 64 && gj > pow2_ceil(n))) break;
+    const int waves = (P * gj + 63) / 64;
+    const int per_simd = (waves + 3) / 4;
+    const int instr = ((n + gj - 1) / gj) * 35 + ((r + gj - 1) / gj) * 15 + 150;
+    const int cost = per_simd * instr;
+    if (cost <= best_cost) { best_cost = cost; best = gj; }
+  }
+  return best;
+}
+
 template <int NT, bool WL>
 __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int r,
                                             double tol2, double abs_tol2, int max_sweeps,
@@ -2014,7 +2033,7 @@ __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int
   const int tid = threadIdx.x;
   const int re = r + (r & 1);
   const int P = re / 2;
-  const int GJ = min(min(NT / pow2_ceil(P), 64), pow2_ceil(n));
+  const int GJ = small_pair_lanes<NT>(P, n, r);
   const int q = tid / GJ, gj = tid % GJ;
   const int L = re > 1 ? re - 1 : 1;
   // a wavefront whose lane groups all lie beyond the last pair only takes
