@@ -1964,9 +1964,7 @@ build_rh_batch_kernel(const RhArgs* pa) {
 // singular values, order and cut. Barriers are workgroup barriers only;
 // column pairs (and QR columns) are reduced by lane groups with wavefront
 // shuffles. Y stays in LDS; the accumulated rotation W (r x r) is kept in LDS
-// when Y and W fit in the dynamic LDS (a.lds_bytes), else in a.W, whose rows
-// a lane group loads at the start of its round so the loads overlap the
-// pair's inner products.
+// when Y and W fit in the dynamic LDS (a.lds_bytes), else in a.W.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double group_sum(double v, int G) { return group_sum1(v, G); }
 
@@ -1991,20 +1989,6 @@ __device__ __forceinline__ double block_sum_nt(double v, double (*slot)[NT / 64]
   return r;
 }
 
-// Cyclic one-sided Jacobi sweeps on the columns of Ys (n x r, ld n, LDS),
-// rotations accumulated in W (r x r, ld r): in LDS after Y when WL, else Wg.
-// Lane group q (GJ lanes) owns pair q of each round; a lane holds rows
-// gj, gj + GJ, ... (at most RP) of the pair's two columns in registers. The
-// row loops are branch-free: a block of GJ rows is skipped when it starts
-// past the end (a test uniform over the workgroup), and inside the last block
-// a lane past the end works on the last row instead, masked out of the inner
-// products; it then stores the same bits as the lane that owns that row.
-// A pair is skipped when neither column has been rotated since the round
-// one sweep earlier (the order repeats every sweep, so that round checked
-// this pair and found it converged): recomputing it would give the same bits.
-// last_rot[c] (LDS) is the round of column c's last rotation; a wavefront
-// whose pairs are all skipped only takes part in the barrier.
-// Returns the sweep count; `converged` when the last sweep rotated nothing.
 // Lanes per pair (a power of two, NT/64..64, all pairs in one pass): the
 // count that minimises the issue slots of the most loaded SIMD, busy
 // wavefronts per SIMD x instructions per wavefront (rows per lane of the two
@@ -2024,6 +2008,20 @@ __device__ __forceinline__ int small_pair_lanes(int P, int n, int r) {
   return best;
 }
 
+// Cyclic one-sided Jacobi sweeps on the columns of Ys (n x r, ld n, LDS),
+// rotations accumulated in W (r x r, ld r): in LDS after Y when WL, else Wg.
+// Lane group q (GJ lanes) owns pair q of each round; a lane holds rows
+// gj, gj + GJ, ... (at most RP) of the pair's two columns in registers. The
+// row loops are branch-free: a block of GJ rows is skipped when it starts
+// past the end (a test uniform over the workgroup), and inside the last block
+// a lane past the end works on the last row instead, masked out of the inner
+// products; it then stores the same bits as the lane that owns that row.
+// A pair is skipped when neither column has been rotated since the round
+// one sweep earlier (the order repeats every sweep, so that round checked
+// this pair and found it converged): recomputing it would give the same bits.
+// last_rot[c] (LDS) is the round of column c's last rotation; a wavefront
+// whose pairs are all skipped only takes part in the barrier.
+// Returns the sweep count; `converged` when the last sweep rotated nothing.
 template <int NT, bool WL>
 __device__ __forceinline__ int small_jacobi(double2* Ys, double2* Wg, int n, int r,
                                             double tol2, double abs_tol2, int max_sweeps,
